@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Benchmark of the structured-mesh finite-volume residual + Runge-Kutta path on one MI355X.
+
+Metric: cell-stage updates/s (cells x RK stages x steps / wall time of the time loop),
+plus the fraction of the HBM roofline of the dominant kernel (the fused stage kernel)
+and the error of the mixed FP32-face / FP64-state policy against the FP64 path.
+
+Default workload: the cylinder O-grid Euler case, 512 (theta) x 256 (r) cells, Roe +
+Harten fix, MinMod2, LSQ gradients, classical RK4 with local time stepping at CFL 1,
+freestream M = 0.3 (synthetic initial condition: uniform freestream).  One "step" is
+one RK4 time step (4 stages) over all cells.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c2|c3|c1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+
+
+def rel_err_conserved(q, qref, law, basis):
+    """max over conserved variables of max|dq| / max|qref| (normwise)."""
+    from hyperboliconvergence_amd import cases
+    if law == 1 and basis == 1:
+        q, qref = cases.prim_to_cons(q), cases.prim_to_cons(qref)
+    errs = []
+    for v in range(q.shape[-1]):
+        s = np.max(np.abs(qref[..., v]))
+        d = np.max(np.abs(q[..., v] - qref[..., v]))
+        errs.append(d / s if s > 0 else d)
+    return float(max(errs))
+
+
+def make_cfg(name, steps):
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    if name == "c1":
+        cfg = cases.config_c1(128)
+        g = 0.0
+    elif name == "c3":
+        cfg = cases.config_c3(4096, hca.FLUX_SLAU)
+        g = 0.0
+    else:
+        cfg = cases.config_c2()
+        g = 16.0
+    cfg["geom_bytes"] = g
+    return cfg
+
+
+def run_gpu(cfg, precision, steps, warmup, graph_steps):
+    import hyperboliconvergence_amd as hca
+    ctx = hca.Context(0)
+    ctx.set_mesh(cfg["nodes"])
+    ctx.set_case(cfg["case"])
+    ctx.set_precision(precision)
+    ctx.set_boundary_refs(cfg["bref"])
+    ctx.set_state(cfg["q"])
+    ctx.set_graph_steps(graph_steps)
+    if warmup:
+        ctx.step(warmup)
+        ctx.synchronize()
+        ctx.set_state(cfg["q"])
+        ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.step(steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    q = ctx.get_state()
+    st = ctx.status()
+    # live kernel timing with HIP events on the context's stream (no graphs while profiling)
+    ctx.set_state(cfg["q"])
+    ctx.set_profiling(True)
+    ctx.step(min(steps, 50))
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    ctx.close()
+    return t1 - t0, q, st, prof
+
+
+def cpu_baseline(cfg, budget_s=12.0):
+    """The oracle's -O3 -ffast-math OpenMP build on the host cores, bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    po.set_threads(threads, fast=True)
+    cs = cfg["case"]
+    ni, nj = cfg["nodes"].shape[0] - 1, cfg["nodes"].shape[1] - 1
+    ns = cs.nstages
+    t0 = time.perf_counter()
+    po.integrate(cs, cfg["nodes"], cfg["q"], 1, cfg["bref"], fast=True)
+    per_step = max(time.perf_counter() - t0, 1e-6)
+    nsteps = int(max(1, min(cfg["nsteps"], budget_s / per_step)))
+    best = None
+    t0 = time.perf_counter()
+    _, _, tm = po.integrate(cs, cfg["nodes"], cfg["q"], nsteps, cfg["bref"], fast=True)
+    el = time.perf_counter() - t0
+    loop_s = tm.sum() / 1e3 if tm.sum() > 0 else el
+    best = loop_s
+    rate = ni * nj * ns * nsteps / best
+    return {"value": rate, "unit": "cell-stage updates/s", "cores": threads, "kind": "port",
+            "sample": f"{cfg['name']}: full grid {ni}x{nj}, {nsteps} of {cfg['nsteps']} steps, oracle built "
+                      f"-O3 -ffast-math -fopenmp (cpp/Makefile:38 flags), {threads} OpenMP threads, "
+                      f"steady_clock around the time loop"}
+
+
+def traffic_from_profiles(workload, precision):
+    """HBM bytes per stage-kernel launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(f"{workload}:{'mixed' if precision else 'fp64'}")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--graph-steps", type=int, default=50)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if "RANK" in os.environ and int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = make_cfg(a.workload, a.steps)
+    steps = a.steps if a.steps is not None else cfg["nsteps"]
+    cs = cfg["case"]
+    ni, nj = cfg["nodes"].shape[0] - 1, cfg["nodes"].shape[1] - 1
+    N = ni * nj
+    ns = cs.nstages
+    nv = 4 if cs.law == 1 else 3
+    nv_evolved = nv if cs.law == 1 else 1
+    bytes_per_cell_stage = 3 * nv_evolved * 8 + cfg["geom_bytes"]
+
+    res = {}
+    for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
+        el, q, st, prof = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
+        kern_ms = prof[0] / max(prof[1], 1.0)
+        res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms, upd_ms=prof[2] / max(prof[3], 1.0),
+                         rate=N * ns * steps / el)
+        print(f"# {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, "
+              f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}",
+              file=sys.stderr, flush=True)
+
+    mixed_err = None
+    if 0 in res and 1 in res:
+        mixed_err = rel_err_conserved(res[1]["q"], res[0]["q"], cs.law, cs.basis)
+        print(f"# mixed vs fp64 state error after {steps} steps: {mixed_err:.3e}", file=sys.stderr, flush=True)
+    head = 1 if (1 in res and (mixed_err is None or mixed_err <= 1e-6)) else 0
+    h = res[head]
+
+    import hyperboliconvergence_amd as hca
+    with hca.Context(0) as c:
+        triad = c.stream_triad(1 << 26, 20)
+    alg_bytes = bytes_per_cell_stage * N
+    achieved = alg_bytes / (h["kernel_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload, head),
+                "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bytes_per_cell_stage,
+                "kernel_us": h["kernel_ms"] * 1e3}
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(cfg)
+        print(f"# cpu baseline: {cpu['value']:.4e} cell-stage/s on {cpu['cores']} threads", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": "cell-stage updates/s + %HBM roofline; mixed-FP32 vs FP64 at ≤1e-6 rel err",
+        "value": h["rate"],
+        "unit": "cell-stage updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": a.warmup,
+        "ms_per_step": h["seconds"] / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-face/f64-state" if head == 1 else "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["name"], "cells": N, "ni": ni, "nj": nj, "rk_stages": ns,
+                   "precision": "mixed" if head == 1 else "fp64", "graph_steps": a.graph_steps},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "fp64_value": res[0]["rate"] if 0 in res else None,
+        "mixed_value": res[1]["rate"] if 1 in res else None,
+        "mixed_vs_fp64_rel_err": mixed_err,
+        "status": int(h["status"]),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,210 @@
+"""MI355X-native structured-mesh finite-volume residual + Runge-Kutta hot path.
+
+The compute path is the in-tree HIP library ``libhcamd.so`` (include/hcamd.h).
+This module is a thin ctypes binding of that C-ABI for Python callers (tests,
+bench, the adjoint wrapper); it contains no numerical fallback: importing
+``Context`` without the built library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(PKG, "libhcamd.so")
+
+LAW_SCALAR, LAW_EULER = 0, 1
+BASIS_CONSERVED, BASIS_PRIMITIVE = 0, 1
+FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE, FLUX_SLAU, FLUX_AUSMPUP = range(5)
+SCALE_CONVECTIVE, SCALE_ACOUSTIC, SCALE_ADAPTIVE = range(3)
+(LIM_NOLIMIT1, LIM_NOLIMIT2, LIM_NOLIMIT3, LIM_VANALBADA2, LIM_MINMOD2, LIM_SUPERBEE2,
+ LIM_MC2, LIM_VANLEER2, LIM_CADA3, LIM_SMOOTHVANALBADA) = range(10)
+BC_PERIODIC, BC_RIEMANN, BC_ENTROPY, BC_FIXED, BC_WALL = range(5)
+DT_GLOBAL, DT_LOCAL = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+STATUS_NONFINITE, STATUS_NONPHYSICAL, STATUS_BCNORMAL = 1, 2, 4
+
+C_ABI_SYMBOLS = (
+    "hc_create", "hc_destroy", "hc_last_error", "hc_set_mesh", "hc_set_case", "hc_set_precision",
+    "hc_set_boundary_refs", "hc_set_state", "hc_get_state", "hc_residual", "hc_step", "hc_get_time",
+    "hc_norms", "hc_get_status", "hc_synchronize", "hc_set_graph_steps", "hc_set_profiling",
+    "hc_get_profile", "hc_device_state", "hc_stream_triad", "hc_version",
+)
+
+
+class Species(C.Structure):
+    """Species<Euler,Real> (conservationLaws/euler/euler.h:48-60)."""
+    _fields_ = [(n, C.c_double) for n in ("gamma", "minf", "lref", "nu", "pr", "dt", "R", "gamma1")]
+
+
+class CaseDesc(C.Structure):
+    """hc_case_desc of include/hcamd.h."""
+    _fields_ = [("law", C.c_int), ("basis", C.c_int), ("flux", C.c_int), ("vscale", C.c_int),
+                ("pscale", C.c_int), ("limiter", C.c_int), ("bc", C.c_int * 4),
+                ("nstages", C.c_int), ("dt_mode", C.c_int),
+                ("alpha", (C.c_double * 6) * 6), ("beta", C.c_double * 6), ("cfl", C.c_double),
+                ("period_i", C.c_double * 2), ("period_j", C.c_double * 2), ("sp", Species)]
+
+
+class HCError(RuntimeError):
+    pass
+
+
+_LIB = None
+
+
+def library():
+    """The loaded libhcamd.so; raises if it has not been built (no fallback path)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIBPATH):
+        raise HCError(f"{LIBPATH} is missing: build it with `python -m hyperboliconvergence_amd.build`")
+    L = C.CDLL(LIBPATH)
+    P, i, d = C.c_void_p, C.c_int, C.c_double
+    sig = {
+        "hc_create": (i, [C.POINTER(P), i]),
+        "hc_destroy": (i, [P]),
+        "hc_last_error": (C.c_char_p, [P]),
+        "hc_set_mesh": (i, [P, i, i, P]),
+        "hc_set_case": (i, [P, P]),
+        "hc_set_precision": (i, [P, i]),
+        "hc_set_boundary_refs": (i, [P, P]),
+        "hc_set_state": (i, [P, P]),
+        "hc_get_state": (i, [P, P]),
+        "hc_residual": (i, [P, P]),
+        "hc_step": (i, [P, i]),
+        "hc_get_time": (i, [P, P]),
+        "hc_norms": (i, [P, P, P]),
+        "hc_get_status": (i, [P, P]),
+        "hc_synchronize": (i, [P]),
+        "hc_set_graph_steps": (i, [P, i]),
+        "hc_set_profiling": (i, [P, i]),
+        "hc_get_profile": (i, [P, P]),
+        "hc_device_state": (i, [P, P]),
+        "hc_stream_triad": (i, [P, C.c_longlong, i, P]),
+        "hc_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _LIB = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Context:
+    """One device context of the C-ABI (hc_create ... hc_destroy)."""
+
+    def __init__(self, device=0):
+        self._L = library()
+        self._h = C.c_void_p()
+        rc = self._L.hc_create(C.byref(self._h), device)
+        if rc != 0:
+            raise HCError(f"hc_create failed ({rc})")
+        self.ni = self.nj = self.nv = 0
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self._L.hc_last_error(self._h)
+            raise HCError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self._h:
+            self._L.hc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_mesh(self, nodes):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+        self._check(self._L.hc_set_mesh(self._h, ni, nj, _p(nodes)), "hc_set_mesh")
+        self.ni, self.nj = ni, nj
+
+    def set_case(self, desc):
+        self._check(self._L.hc_set_case(self._h, C.cast(C.pointer(desc), C.c_void_p)), "hc_set_case")
+        self.nv = 4 if desc.law == LAW_EULER else 3
+
+    def set_precision(self, precision):
+        self._check(self._L.hc_set_precision(self._h, int(precision)), "hc_set_precision")
+
+    def set_boundary_refs(self, bref):
+        bref = np.ascontiguousarray(bref, dtype=np.float64)
+        assert bref.size == (2 * self.nj + 2 * self.ni) * self.nv
+        self._check(self._L.hc_set_boundary_refs(self._h, _p(bref)), "hc_set_boundary_refs")
+
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == self.ni * self.nj * self.nv
+        self._check(self._L.hc_set_state(self._h, _p(q)), "hc_set_state")
+
+    def get_state(self):
+        q = np.zeros((self.ni, self.nj, self.nv))
+        self._check(self._L.hc_get_state(self._h, _p(q)), "hc_get_state")
+        return q
+
+    def residual(self):
+        r = np.zeros((self.ni, self.nj, self.nv + 1))
+        self._check(self._L.hc_residual(self._h, _p(r)), "hc_residual")
+        return r
+
+    def step(self, nsteps):
+        self._check(self._L.hc_step(self._h, int(nsteps)), "hc_step")
+
+    def synchronize(self):
+        self._check(self._L.hc_synchronize(self._h), "hc_synchronize")
+
+    def time(self):
+        t = C.c_double()
+        self._check(self._L.hc_get_time(self._h, C.byref(t)), "hc_get_time")
+        return t.value
+
+    def norms(self):
+        l2, linf = np.zeros(self.nv), np.zeros(self.nv)
+        self._check(self._L.hc_norms(self._h, _p(l2), _p(linf)), "hc_norms")
+        return l2, linf
+
+    def status(self):
+        s = C.c_uint()
+        self._check(self._L.hc_get_status(self._h, C.byref(s)), "hc_get_status")
+        return s.value
+
+    def set_graph_steps(self, k):
+        self._check(self._L.hc_set_graph_steps(self._h, int(k)), "hc_set_graph_steps")
+
+    def set_profiling(self, on):
+        self._check(self._L.hc_set_profiling(self._h, 1 if on else 0), "hc_set_profiling")
+
+    def profile(self):
+        out = np.zeros(4)
+        self._check(self._L.hc_get_profile(self._h, _p(out)), "hc_get_profile")
+        return out
+
+    def stream_triad(self, n, reps):
+        g = C.c_double()
+        self._check(self._L.hc_stream_triad(self._h, int(n), int(reps), C.byref(g)), "hc_stream_triad")
+        return g.value
+
+    def device_state_ptr(self):
+        p = C.c_void_p()
+        self._check(self._L.hc_device_state(self._h, C.byref(p)), "hc_device_state")
+        return p.value
+
+
+def version():
+    return library().hc_version().decode()

@@ -1,0 +1,233 @@
+"""Case/driver layer: meshes, initial conditions, Runge-Kutta tables and case
+descriptors of the benchmark configurations.
+
+Mesh generators restate make_linspace_mesh / make_cylinder_mesh
+(mesh/generate/twoD.h:178-203, 302-342) and add the geometrically stretched
+O-grid of the cylinder cases.  Runge-Kutta tables are ODE::Explicit::ssp11/22/
+33/34 (ode.h:107-197) plus classical RK4 in the same (alpha, beta) form.
+"""
+import math
+
+import numpy as np
+
+from . import (BASIS_CONSERVED, BASIS_PRIMITIVE, BC_PERIODIC, BC_RIEMANN, BC_WALL, DT_GLOBAL, DT_LOCAL,
+               FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MINMOD2,
+               LIM_SMOOTHVANALBADA, LIM_VANALBADA2, LIM_VANLEER2, SCALE_ACOUSTIC, SCALE_CONVECTIVE, CaseDesc,
+               Species)
+
+GAMMA = 1.4
+
+
+def air_species(minf=None):
+    """get_air_species<double>() (conservationLaws/euler/species.ipp:2-18)."""
+    s = Species()
+    s.gamma = 1.4
+    s.minf = 0.0001 if minf is None else minf
+    s.lref = 1.0
+    s.pr = 0.7
+    s.nu = 1.81e-5
+    s.dt = np.finfo(np.float64).max
+    s.R = 287.058
+    s.gamma1 = 1.0 / (s.gamma - 1.0)
+    return s
+
+
+def rk_table(name):
+    """(nstages, alpha[6][6], beta[6]) of ssp11, ssp22, ssp33, ssp34 (ode.h:107-197) or rk4."""
+    a = np.zeros((6, 6))
+    b = np.zeros(6)
+    if name == "ssp11":
+        n = 1
+        b[0] = 1.0
+        a[0, 0] = 1.0
+    elif name == "ssp22":
+        n = 2
+        b[:2] = (1.0, 1.0)
+        a[0, 0] = 1.0
+        a[1, :2] = (0.5, 0.5)
+    elif name == "ssp33":
+        n = 3
+        b[:3] = (1.0, 0.5, 1.0)
+        a[0, 0] = 1.0
+        a[1, :2] = (0.5, 0.5)
+        a[2, :3] = (1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0)
+    elif name == "ssp34":
+        n = 4
+        b[:4] = (0.5, 1.0, 0.5, 1.0)
+        a[0, 0] = 1.0
+        a[1, :2] = (0.5, 0.5)
+        a[2, :3] = (1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0)
+        a[3, :4] = (1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 2.0)
+    elif name == "rk4":
+        # classical RK4: q_{s+1} = q0 + beta_s dt sum_k alpha[s][k] R_k
+        n = 4
+        b[:4] = (0.5, 0.5, 1.0, 1.0)
+        a[0, 0] = 1.0
+        a[1, 1] = 1.0
+        a[2, 2] = 1.0
+        a[3, :4] = (1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0)
+    else:
+        raise ValueError(name)
+    return n, a, b
+
+
+def make_case(law, basis, flux, limiter, bc, rk, cfl, dt_mode=DT_GLOBAL, species=None, period_i=(0.0, 0.0),
+              period_j=(0.0, 0.0), vscale=None, pscale=None):
+    cs = CaseDesc()
+    cs.law, cs.basis, cs.flux, cs.limiter = law, basis, flux, limiter
+    cs.vscale = SCALE_CONVECTIVE if vscale is None else vscale
+    cs.pscale = (SCALE_ACOUSTIC if flux == FLUX_SLAU else SCALE_CONVECTIVE) if pscale is None else pscale
+    for k in range(4):
+        cs.bc[k] = bc[k]
+    n, a, b = rk_table(rk)
+    cs.nstages = n
+    for s in range(6):
+        cs.beta[s] = b[s]
+        for k in range(6):
+            cs.alpha[s][k] = a[s, k]
+    cs.cfl = cfl
+    cs.dt_mode = dt_mode
+    cs.period_i[0], cs.period_i[1] = period_i
+    cs.period_j[0], cs.period_j[1] = period_j
+    cs.sp = species if species is not None else air_species()
+    return cs
+
+
+# ---------------- meshes ----------------
+
+def mesh_linspace(ni, nj, lox, hix, loy, hiy):
+    """make_linspace_mesh (mesh/generate/twoD.h:178-203): nodes [ni+1][nj+1][2]."""
+    dx = (hix - lox) / ni
+    dy = (hiy - loy) / nj
+    i = np.arange(ni + 1, dtype=np.float64)[:, None]
+    j = np.arange(nj + 1, dtype=np.float64)[None, :]
+    nodes = np.empty((ni + 1, nj + 1, 2))
+    nodes[..., 0] = lox + i * dx
+    nodes[..., 1] = loy + j * dy
+    return nodes
+
+
+def ogrid_ratio(nr, rc=0.5, rb=20.0, dr0=1e-3):
+    """Geometric ratio s with dr0 (s^nr - 1)/(s - 1) = rb - rc (bisection)."""
+    L = rb - rc
+
+    def g(s):
+        return dr0 * (s ** nr - 1.0) / (s - 1.0) - L
+
+    lo, hi = 1.0 + 1e-12, 2.0
+    while g(hi) < 0:
+        hi *= 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if mid <= lo or mid >= hi:
+            break
+        if g(mid) < 0:
+            lo = mid
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def mesh_ogrid(nr, nth, rc=0.5, rb=20.0, dr0=1e-3):
+    """O-grid around a cylinder, i = radius (geometric stretching, first spacing dr0),
+    j = theta (uniform, node nth identical to node 0), as make_cylinder_mesh's layout
+    (mesh/generate/twoD.h:302-342)."""
+    s = ogrid_ratio(nr, rc, rb, dr0)
+    i = np.arange(nr + 1)
+    r = rc + dr0 * (np.power(s, i.astype(np.float64)) - 1.0) / (s - 1.0)
+    r[nr] = rb
+    jj = np.arange(nth + 1)
+    jj[nth] = 0
+    th = (2.0 * math.pi * jj) / nth
+    nodes = np.empty((nr + 1, nth + 1, 2))
+    nodes[..., 0] = r[:, None] * np.cos(th)[None, :]
+    nodes[..., 1] = r[:, None] * np.sin(th)[None, :]
+    return nodes
+
+
+def cell_centres(nodes):
+    """Vertex-average cell centres (geometry/volume.ipp:64-72)."""
+    p0, p1, p2, p3 = nodes[:-1, :-1], nodes[1:, :-1], nodes[:-1, 1:], nodes[1:, 1:]
+    return (((0.25 * p0 + 0.25 * p1) + 0.25 * p2) + 0.25 * p3)
+
+
+def bref_size(ni, nj):
+    return 2 * nj + 2 * ni
+
+
+# ---------------- Euler helpers ----------------
+
+def prim_to_cons(qp, gamma=GAMMA, R=287.058):
+    """Conserved {rho u, rho v, rho, rho E} from primitive {u, v, rho, p}
+    (euler/transforms/primitive.ipp:235-271 then conserved.ipp:136-163)."""
+    qp = np.asarray(qp, dtype=np.float64)
+    u, v, r, p = qp[..., 0], qp[..., 1], qp[..., 2], qp[..., 3]
+    k = 0.5 * (u * u + v * v)
+    t = p / (r * R)
+    a2 = gamma * R * t
+    h = a2 * (1.0 / (gamma - 1.0)) + k
+    return np.stack([r * u, r * v, r, r * h - p], axis=-1)
+
+
+def isentropic_vortex(x, y, t=0.0, beta=5.0, mach=0.5, gamma=GAMMA, x0=0.0, y0=0.0, L=20.0):
+    """Shu's isentropic vortex (rho_inf = p_inf = 1) advected diagonally at Mach `mach`,
+    primitive {u, v, rho, p}; periodic domain of side L centred on the origin."""
+    a_inf = math.sqrt(gamma)
+    uinf = mach * a_inf / math.sqrt(2.0)
+    vinf = uinf
+    xc = (x0 + uinf * t + L / 2) % L - L / 2
+    yc = (y0 + vinf * t + L / 2) % L - L / 2
+    dx = (x - xc + L / 2) % L - L / 2
+    dy = (y - yc + L / 2) % L - L / 2
+    r2 = dx * dx + dy * dy
+    e = np.exp((1.0 - r2) / 2.0)
+    du = -beta / (2.0 * math.pi) * e * dy
+    dv = beta / (2.0 * math.pi) * e * dx
+    T = 1.0 - (gamma - 1.0) * beta * beta / (8.0 * gamma * math.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho ** gamma
+    return np.stack([uinf + du, vinf + dv, rho, p], axis=-1)
+
+
+# ---------------- benchmark configurations ----------------
+
+def config_c1(n=128):
+    """Scalar advection on a periodic n x n grid over [0,1]^2, velocity (1, 0.5), Gaussian
+    bump, Rusanov (= upwind for linear advection), VanLeer2, SSP-RK3, CFL 0.5, FP64."""
+    nodes = mesh_linspace(n, n, 0.0, 1.0, 0.0, 1.0)
+    c = cell_centres(nodes)
+    q = np.empty((n, n, 3))
+    q[..., 0] = 1.0
+    q[..., 1] = 0.5
+    q[..., 2] = np.exp(-((c[..., 0] - 0.5) ** 2 + (c[..., 1] - 0.5) ** 2) / 0.01)
+    cs = make_case(LAW_SCALAR, BASIS_CONSERVED, FLUX_RUSANOV, LIM_VANLEER2, (BC_PERIODIC,) * 4, "ssp33", 0.5,
+                   DT_GLOBAL, Species(), period_i=(1.0, 0.0), period_j=(0.0, 1.0))
+    return dict(name="scalar-advection-128x128-vanleer2-upwind-ssprk3", nodes=nodes, q=q, case=cs,
+                bref=np.zeros((bref_size(n, n), 3)), nsteps=200)
+
+
+def config_c2(nth=512, nr=256, mach=0.3, nsteps=2000):
+    """Euler O-grid around a unit-diameter cylinder, 512 (theta) x 256 (r) cells, r from 0.5 to 20
+    with first spacing 1e-3, freestream M=0.3 (rho=1, p=1/gamma), slip wall (id 0), Riemann far field
+    (id 1), periodic theta (ids 2, 3); Roe + Harten fix, MinMod2, LSQ, RK4 with local dt at CFL 1."""
+    nodes = mesh_ogrid(nr, nth)
+    qinf = np.array([mach, 0.0, 1.0, 1.0 / GAMMA])
+    q = np.broadcast_to(qinf, (nr, nth, 4)).copy()
+    bref = np.broadcast_to(qinf, (bref_size(nr, nth), 4)).copy()
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, FLUX_ROE, LIM_MINMOD2, (BC_WALL, BC_RIEMANN, BC_PERIODIC, BC_PERIODIC),
+                   "rk4", 1.0, DT_LOCAL, air_species(mach))
+    return dict(name=f"euler-cylinder-ogrid-{nth}x{nr}-roe-minmod-lsq-rk4-local", nodes=nodes, q=q, case=cs,
+                bref=bref, nsteps=nsteps)
+
+
+def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):
+    """Isentropic vortex on a periodic n x n grid over [-10,10]^2, M=0.5 diagonal, strength 5;
+    SLAU or AUSM+up, VanAlbada2, SSP-RK3, CFL 0.8."""
+    nodes = mesh_linspace(n, n, -10.0, 10.0, -10.0, 10.0)
+    c = cell_centres(nodes)
+    q = isentropic_vortex(c[..., 0], c[..., 1])
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, flux, LIM_VANALBADA2, (BC_PERIODIC,) * 4, "ssp33", 0.8, DT_GLOBAL,
+                   air_species(0.5), period_i=(20.0, 0.0), period_j=(0.0, 20.0))
+    fname = "slau" if flux == FLUX_SLAU else "ausmpup"
+    return dict(name=f"euler-isentropic-vortex-{n}x{n}-{fname}-vanalbada-ssprk3", nodes=nodes, q=q, case=cs,
+                bref=np.zeros((bref_size(n, n), 4)), nsteps=nsteps)

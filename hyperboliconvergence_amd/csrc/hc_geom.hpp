@@ -1,0 +1,32 @@
+// hc_geom.hpp -- one-time geometry of a structured mesh for the stage kernel:
+// cell volumes and centres (geometry/volume.ipp:55-96, mesh/dual.ipp:121-162),
+// face surfaces (geometry/surface.ipp:12-29), the LSQ spatial metric
+// (spatial/lsqMetrics.h:119-251) and, per face, everything of the MUSCL
+// reconstruction that depends on geometry only (spatial/muscl.h:35-50,
+// lsq/factor_solve.ipp:403-490): distances d_L, d_R, 1/(d_L+d_R), the centre
+// difference dx_c and the factored left/right biased metrics.
+#pragma once
+
+#include <vector>
+
+namespace hc {
+
+struct GeomInput {
+    int ni, nj;
+    const double* nodes;   // [ni+1][nj+1][2]
+    int bc[4];             // boundary ids 0..3 (0 = periodic)
+    double period_i[2];    // translation of a cell to its periodic image across boundary 1
+    double period_j[2];    // ... across boundary 3
+};
+
+struct GeomOutput {
+    std::vector<double> cvol, ccx, ccy;   // [ni*nj]
+    std::vector<double> X;                // [ni*nj][3]  (kept for inspection)
+    std::vector<double> fi;               // [NSLOT][(ni+1)*nj]
+    std::vector<double> fj;               // [NSLOT][ni*(nj+1)]
+};
+
+// returns 0 on success, -1 on a degenerate cell or face
+int build_geometry(const GeomInput& in, GeomOutput& out);
+
+}   // namespace hc

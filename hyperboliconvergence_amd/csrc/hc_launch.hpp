@@ -1,0 +1,33 @@
+// hc_launch.hpp -- launchers of the stage kernels, one per (law, basis, flux,
+// precision) instantiation (hc_kernel_inst.hip is compiled once per combination).
+#pragma once
+
+#include "hc_stage.hpp"
+
+namespace hc {
+
+// tile shapes: FP64 face arithmetic 4x32 (128 lanes), mixed 8x32 (256 lanes)
+constexpr int TI64 = 4, TJ64 = 32;
+constexpr int TI32 = 8, TJ32 = 32;
+
+using StageLaunch64 = hipError_t (*)(const StageArgs<double>&, hipStream_t);
+using StageLaunch32 = hipError_t (*)(const StageArgs<float>&, hipStream_t);
+
+}   // namespace hc
+
+#define HC_FOR_EACH_KERNEL(X) \
+    X(0, 0, 0) X(0, 0, 1)     \
+    X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) \
+    X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 1, 3) X(1, 1, 4)
+
+#define HC_FOR_EACH_LAWBASIS(X) X(0, 0) X(1, 0) X(1, 1)
+
+#define HC_DECL_STAGE(L, B, F)                                                                     \
+    hipError_t hc_stage_L##L##_B##B##_F##F##_P64(const hc::StageArgs<double>&, hipStream_t);       \
+    hipError_t hc_stage_L##L##_B##B##_F##F##_P32(const hc::StageArgs<float>&, hipStream_t);
+#define HC_DECL_UPD(L, B)                                                                          \
+    hipError_t hc_upd_L##L##_B##B##_P64(const hc::StageArgs<double>&, hipStream_t);                \
+    hipError_t hc_upd_L##L##_B##B##_P32(const hc::StageArgs<float>&, hipStream_t);
+
+HC_FOR_EACH_KERNEL(HC_DECL_STAGE)
+HC_FOR_EACH_LAWBASIS(HC_DECL_UPD)

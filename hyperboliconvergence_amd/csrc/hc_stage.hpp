@@ -1,0 +1,571 @@
+// hc_stage.hpp -- the fused residual + Runge-Kutta stage of the structured-mesh
+// finite-volume path, as one tiled kernel per stage.
+//
+// One work-group owns a TI x TJ tile of cells (j, the contiguous index, along
+// the 64-lane wavefront).  Phases, separated by work-group barriers:
+//   1. load   : solution variables of the tile plus a 2-cell halo into LDS
+//               (periodic wrap across periodic boundaries)
+//   2. qmetric: LSQ solution metric Q = sum dq (x) dx for the tile plus a
+//               1-cell ring (spatial/lsqMetrics.h:269-301)
+//   3. faces  : every i- and j-face touching the tile: MUSCL reconstruction
+//               with LSQ-biased gradients and the limiter, then the flux
+//               functor (spatial/muscl.h:10-87); boundary faces apply the ghost
+//               value / wall flux of their boundary condition
+//   4. cells  : owner-computes accumulation of the cell residual in the
+//               reference's order (spatial/residualCalc.h:199-305), no atomics,
+//               then the Runge-Kutta accumulation and forward-Euler update
+//               (timestepping/rungeKutta.h:176-197, spatial/eulerForwardUpdate.h)
+// The phases are __host__ __device__ so that a work-group can be executed
+// thread by thread on the host for debugging; the kernel wrappers are at the end.
+#pragma once
+
+#include "hc_point.hpp"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+#include <cstdint>
+#include <cstring>
+
+namespace hc {
+
+// per-face geometry slots (slot-major arrays, one plane per slot)
+enum FaceSlot {
+    FS_AREA = 0, FS_N0, FS_N1, FS_DL, FS_DR, FS_S1, FS_DX0, FS_DX1,
+    FS_FL0, FS_FL1, FS_FL2, FS_FR0, FS_FR1, FS_FR2, NSLOT
+};
+// boundary (non-periodic) faces store the face centre in FS_DL / FS_DR
+constexpr int FS_CX = FS_DL;
+constexpr int FS_CY = FS_DR;
+
+// status bits returned through the C-ABI
+enum StatusBits : unsigned int {
+    ST_NONFINITE = 1u,     // non-finite state after an update
+    ST_NONPHYSICAL = 2u,   // non-positive density or pressure (Euler)
+    ST_BCNORMAL = 4u       // far-field boundary face normal does not point into the domain
+};
+
+template <class RF> struct MeshView {
+    int ni, nj, per_i, per_j;
+    long long nfi, nfj;   // number of i-faces (ni+1)*nj and j-faces ni*(nj+1)
+    const double* cvol;
+    const double* ccx;
+    const double* ccy;
+    const RF* fi;   // [NSLOT][nfi], i-face at node line l, row j: l*nj + j
+    const RF* fj;   // [NSLOT][nfj], j-face at row i, node line l: i*(nj+1) + l
+};
+
+// Runge-Kutta accumulation plan of one stage:
+//   R_tot = (base >= 0 ? A[base] : +0) + R_s*alpha[s][s]
+//   A[slot] = (first ? +0 : A[slot]) + R_s*alpha[r][s]   for later rows r
+struct RkPlan {
+    int base;
+    int nupd;
+    int upd_slot[5];
+    int upd_first[5];
+    double a_ss;
+    double upd_coef[5];
+    double beta;
+};
+
+enum StageMode { MODE_FUSED = 0, MODE_GLOBAL_STAGE0 = 1, MODE_RESIDUAL = 2 };
+
+// Accumulation plans of a Runge-Kutta table in the reference's (alpha, beta) form
+// (ode.h:87-105): one accumulator per later row with a non-zero coefficient
+// before its diagonal.  Returns the number of accumulators.
+template <class Vec> inline int make_rk_plans(int ns, const double (*alpha)[6], const double* beta, Vec& plans)
+{
+    int slot[6] = {-1, -1, -1, -1, -1, -1};
+    int nslots = 0;
+    for (int r = 1; r < ns; ++r) {
+        bool any = false;
+        for (int k = 0; k < r; ++k) any = any || alpha[r][k] != 0.0;
+        if (any) slot[r] = nslots++;
+    }
+    plans.assign(ns, RkPlan{});
+    for (int s = 0; s < ns; ++s) {
+        RkPlan& p = plans[s];
+        p.base = slot[s];
+        p.a_ss = alpha[s][s];
+        p.beta = beta[s];
+        p.nupd = 0;
+        for (int r = s + 1; r < ns; ++r) {
+            if (slot[r] < 0 || alpha[r][s] == 0.0) continue;
+            bool first = true;
+            for (int k = 0; k < s; ++k) first = first && alpha[r][k] == 0.0;
+            p.upd_slot[p.nupd] = slot[r];
+            p.upd_first[p.nupd] = first ? 1 : 0;
+            p.upd_coef[p.nupd] = alpha[r][s];
+            p.nupd++;
+        }
+    }
+    return nslots;
+}
+
+template <class RF> struct StageArgs {
+    MeshView<RF> m;
+    Species<double> sp;
+    Species<RF> spf;
+    int vscale, pscale, limiter;
+    int bc[4];
+    int mode, dt_mode, stage, reset_max;
+    long long N;
+    double cfl;
+    const double* q_in;   // stage state [nv][N]
+    const double* q0;     // step-start state [nv][N]
+    double* q_out;        // [nv][N]
+    double* acc;          // accumulators [slot][nv][N]
+    double* tmp;          // deferred R_tot of global-dt stage 0 [nv][N]
+    double* lam0;         // stage-0 spectral radius sums [N]
+    const double* bref;   // boundary reference values, AoS [2nj+2ni][nv]
+    double* res_out;      // residual-only mode: AoS [N][nv+1]
+    unsigned long long* maxslot;   // bits of max(lambda/vol), reset to bits(DBL_MIN)
+    double* dtslot;       // [0] dt of the current step, [1] accumulated time
+    unsigned int* status;
+    RkPlan plan;
+};
+
+HC_HD unsigned long long dbl_bits(double x)
+{
+    unsigned long long u;
+    std::memcpy(&u, &x, sizeof(u));
+    return u;
+}
+HC_HD double bits_dbl(unsigned long long u)
+{
+    double x;
+    std::memcpy(&x, &u, sizeof(x));
+    return x;
+}
+
+HC_HD bool hc_isfinite(double x) { return x - x == 0.0; }
+
+// Bijection of work-group ids onto tiles such that the ids an XCD receives
+// (b % 8 == x) cover one contiguous band of tiles.
+HC_HD int xcd_tile(int b, int ntiles)
+{
+    const int q = ntiles / 8, r = ntiles % 8;
+    const int x = b % 8, y = b / 8;
+    const int start = x * q + (x < r ? x : r);
+    return start + y;
+}
+
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTile {
+    static constexpr int NV = LawTraits<LAW>::nv;
+    static constexpr int NT = TI * TJ;
+    static constexpr int QH = TI + 4, QW = TJ + 4;   // q tile with 2-cell halo
+    static constexpr int MH = TI + 2, MW = TJ + 2;   // Q ring with 1-cell halo
+    static constexpr int NFI = (TI + 1) * TJ;
+    static constexpr int NFJ = TI * (TJ + 1);
+
+    struct Smem {
+        RF q[NV][QH][QW];
+        RF Q[NV][2][MH][MW];
+        RF Fi[NV + 1][TI + 1][TJ];
+        RF Fj[NV + 1][TI][TJ + 1];
+        double red[NT];
+        unsigned int flg[NT];
+    };
+
+    HC_HD static int wrapi(int g, int n, int per)
+    {
+        if (per) return ((g % n) + n) % n;
+        return g < 0 ? 0 : (g >= n ? n - 1 : g);
+    }
+
+    // ---- phase 1: solution tile with a 2-cell halo ----
+    HC_HD static void load(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        for (int k = tid; k < QH * QW; k += NT) {
+            const int li = k / QW, lj = k % QW;
+            const int gi = wrapi(i0 - 2 + li, ni, a.m.per_i);
+            const int gj = wrapi(j0 - 2 + lj, nj, a.m.per_j);
+            const long long c = (long long)gi * nj + gj;
+            for (int v = 0; v < NV; ++v) s.q[v][li][lj] = RF(a.q_in[v * a.N + c]);
+        }
+    }
+
+    // ---- phase 2: LSQ solution metric of tile cells and the 1-cell ring ----
+    // per cell: interior i-edges, interior j-edges, periodic i wrap, periodic j wrap
+    // (the oracle's neighbour_accumulation order, parallalg/neighbour_algorithm.h:253-317)
+    HC_HD static void qmetric(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        const RF* fi = a.m.fi;
+        const RF* fj = a.m.fj;
+        const long long nfi = a.m.nfi, nfj = a.m.nfj;
+        for (int k = tid; k < MH * MW; k += NT) {
+            const int li = k / MW, lj = k % MW;
+            const int ri = i0 - 1 + li, rj = j0 - 1 + lj;   // unwrapped global index
+            if (!a.m.per_i && (ri < 0 || ri >= ni)) continue;
+            if (!a.m.per_j && (rj < 0 || rj >= nj)) continue;
+            const int gi = wrapi(ri, ni, a.m.per_i);
+            const int gj = wrapi(rj, nj, a.m.per_j);
+            const int r = li + 1, c = lj + 1;   // position in the q tile
+            RF acc[NV][2];
+            for (int v = 0; v < NV; ++v) {
+                acc[v][0] = RF(0);
+                acc[v][1] = RF(0);
+            }
+            auto edge = [&](int rl, int cl, int rr, int cr, const RF* base, long long nf, long long fidx) {
+                const RF dx0 = base[FS_DX0 * nf + fidx];
+                const RF dx1 = base[FS_DX1 * nf + fidx];
+                for (int v = 0; v < NV; ++v) {
+                    const RF dq = s.q[v][rr][cr] - s.q[v][rl][cl];
+                    acc[v][0] = acc[v][0] + dq * dx0;
+                    acc[v][1] = acc[v][1] + dq * dx1;
+                }
+            };
+            if (gi > 0) edge(r - 1, c, r, c, fi, nfi, (long long)gi * nj + gj);
+            if (gi < ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)(gi + 1) * nj + gj);
+            if (gj > 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1) + gj);
+            if (gj < nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1) + gj + 1);
+            if (a.m.per_i) {
+                if (gi == 0) edge(r - 1, c, r, c, fi, nfi, (long long)gj);
+                if (gi == ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)gj);
+            }
+            if (a.m.per_j) {
+                if (gj == 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1));
+                if (gj == nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1));
+            }
+            for (int v = 0; v < NV; ++v) {
+                s.Q[v][0][li][lj] = acc[v][0];
+                s.Q[v][1][li][lj] = acc[v][1];
+            }
+        }
+    }
+
+    // MUSCL face flux (spatial/muscl.h:10-87) with precomputed face geometry
+    HC_HD static Flux4<RF> muscl(const StageArgs<RF>& a, const RF* base, long long nf, long long fidx,
+                                 const RF* ql, const RF* qr, const RF (*Ql)[2], const RF (*Qr)[2])
+    {
+        FaceG<RF> f;
+        f.area = base[FS_AREA * nf + fidx];
+        f.n0 = base[FS_N0 * nf + fidx];
+        f.n1 = base[FS_N1 * nf + fidx];
+        if (a.limiter == LIM_NOLIMIT1) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, ql, qr);
+        const RF dl = base[FS_DL * nf + fidx];
+        const RF dr = base[FS_DR * nf + fidx];
+        const RF s1 = base[FS_S1 * nf + fidx];
+        const RF dx0 = base[FS_DX0 * nf + fidx];
+        const RF dx1 = base[FS_DX1 * nf + fidx];
+        const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
+        const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
+        RF qlf[NV], qrf[NV];
+        for (int v = 0; v < NV; ++v) {
+            const RF dq = qr[v] - ql[v];
+            const RF dqdx = dq * s1;
+            const RF c0 = dq * dx0;
+            const RF c1 = dq * dx1;
+            const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
+            const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
+            const RF slope_l = limit(a.limiter, dqdx, gl);
+            const RF slope_r = limit(a.limiter, dqdx, gr);
+            qlf[v] = ql[v] + slope_l * dl;
+            qrf[v] = qr[v] - slope_r * dr;
+        }
+        return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, qlf, qrf);
+    }
+
+    // non-periodic boundary face: ghost-cell flux or wall flux, in the orientation the
+    // cell accumulation expects (ghostCellLoop.h:133-224, boundaryFluxLoop.h:150-261)
+    HC_HD static Flux4<RF> boundary_face(const StageArgs<RF>& a, int bid, const RF* base, long long nf, long long fidx,
+                                         const RF* qint, long long cint, long long bref_row, unsigned int& flag)
+    {
+        FaceG<RF> fres;
+        fres.area = base[FS_AREA * nf + fidx];
+        fres.n0 = base[FS_N0 * nf + fidx];
+        fres.n1 = base[FS_N1 * nf + fidx];
+        // the update surface faces into the domain: reversed on boundaries 1 and 3
+        FaceG<RF> fupd = fres;
+        if (bid == 1 || bid == 3) {
+            fupd.n0 = -fres.n0;
+            fupd.n1 = -fres.n1;
+        }
+        const int type = a.bc[bid];
+        if constexpr (LAW == LAW_EULER) {
+            if (type == BC_WALL) return euler_wall_flux<RF, BASIS>(a.spf, fupd, qint);
+        }
+        RF qref[NV], qg[NV];
+        for (int v = 0; v < NV; ++v) qref[v] = RF(a.bref[bref_row * NV + v]);
+        const RF fcx = base[FS_CX * nf + fidx];
+        const RF fcy = base[FS_CY * nf + fidx];
+        const RF ccx = RF(a.m.ccx[cint]);
+        const RF ccy = RF(a.m.ccy[cint]);
+        if (!ghost_value<RF, LAW, BASIS>(a.spf, type, fupd, fcx, fcy, ccx, ccy, qint, qref, qg)) flag |= ST_BCNORMAL;
+        if (bid == 0 || bid == 2) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, fres, qg, qint);
+        return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, fres, qint, qg);
+    }
+
+    HC_HD static void store_i(Smem& s, int li, int lj, const Flux4<RF>& F)
+    {
+        for (int v = 0; v < NV; ++v) s.Fi[v][li][lj] = F.f[v];
+        s.Fi[NV][li][lj] = F.lambda;
+    }
+    HC_HD static void store_j(Smem& s, int li, int lj, const Flux4<RF>& F)
+    {
+        for (int v = 0; v < NV; ++v) s.Fj[v][li][lj] = F.f[v];
+        s.Fj[NV][li][lj] = F.lambda;
+    }
+
+    // ---- phase 3: all faces of the tile ----
+    HC_HD static void faces(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        for (int k = tid; k < NFI + NFJ; k += NT) {
+            RF ql[NV], qr[NV], Ql[NV][2], Qr[NV][2];
+            if (k < NFI) {
+                // i-face on node line l between cells (l-1, j) and (l, j)
+                const int li = k / TJ, lj = k % TJ;
+                const int l = i0 + li, j = j0 + lj;
+                if (l > ni || j >= nj) continue;
+                const bool interior = l >= 1 && l <= ni - 1;
+                if (interior || a.m.per_i) {
+                    const long long fidx = (long long)(interior ? l : 0) * nj + j;
+                    for (int v = 0; v < NV; ++v) {
+                        ql[v] = s.q[v][li + 1][lj + 2];
+                        qr[v] = s.q[v][li + 2][lj + 2];
+                        Ql[v][0] = s.Q[v][0][li][lj + 1];
+                        Ql[v][1] = s.Q[v][1][li][lj + 1];
+                        Qr[v][0] = s.Q[v][0][li + 1][lj + 1];
+                        Qr[v][1] = s.Q[v][1][li + 1][lj + 1];
+                    }
+                    store_i(s, li, lj, muscl(a, a.m.fi, a.m.nfi, fidx, ql, qr, Ql, Qr));
+                } else {
+                    const int bid = (l == 0) ? 0 : 1;
+                    const int ic = (l == 0) ? 0 : ni - 1;
+                    const int lr = (l == 0) ? li + 2 : li + 1;
+                    for (int v = 0; v < NV; ++v) ql[v] = s.q[v][lr][lj + 2];
+                    const long long fidx = (long long)l * nj + j;
+                    const long long row = (bid == 0 ? 0 : nj) + j;
+                    store_i(s, li, lj, boundary_face(a, bid, a.m.fi, a.m.nfi, fidx, ql, (long long)ic * nj + j, row, flag));
+                }
+            } else {
+                // j-face on node line l between cells (i, l-1) and (i, l)
+                const int kk = k - NFI;
+                const int li = kk / (TJ + 1), lj = kk % (TJ + 1);
+                const int i = i0 + li, l = j0 + lj;
+                if (i >= ni || l > nj) continue;
+                const bool interior = l >= 1 && l <= nj - 1;
+                if (interior || a.m.per_j) {
+                    const long long fidx = (long long)i * (nj + 1) + (interior ? l : 0);
+                    for (int v = 0; v < NV; ++v) {
+                        ql[v] = s.q[v][li + 2][lj + 1];
+                        qr[v] = s.q[v][li + 2][lj + 2];
+                        Ql[v][0] = s.Q[v][0][li + 1][lj];
+                        Ql[v][1] = s.Q[v][1][li + 1][lj];
+                        Qr[v][0] = s.Q[v][0][li + 1][lj + 1];
+                        Qr[v][1] = s.Q[v][1][li + 1][lj + 1];
+                    }
+                    store_j(s, li, lj, muscl(a, a.m.fj, a.m.nfj, fidx, ql, qr, Ql, Qr));
+                } else {
+                    const int bid = (l == 0) ? 2 : 3;
+                    const int jc = (l == 0) ? 0 : nj - 1;
+                    const int lc = (l == 0) ? lj + 2 : lj + 1;
+                    for (int v = 0; v < NV; ++v) ql[v] = s.q[v][li + 2][lc];
+                    const long long fidx = (long long)i * (nj + 1) + l;
+                    const long long row = 2LL * nj + (bid == 2 ? 0 : ni) + i;
+                    store_j(s, li, lj, boundary_face(a, bid, a.m.fj, a.m.nfj, fidx, ql, (long long)i * nj + jc, row, flag));
+                }
+            }
+        }
+    }
+
+    // ---- phase 4: owner-computes residual accumulation and the stage update ----
+    HC_HD static void cells(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        const int li = tid / TJ, lj = tid % TJ;
+        const int i = i0 + li, j = j0 + lj;
+        s.red[tid] = 0.0;
+        if (i >= ni || j >= nj) {
+            s.flg[tid] = flag;
+            return;
+        }
+        double R[NV];
+        double lam = 0.0;
+        for (int v = 0; v < NV; ++v) R[v] = 0.0;
+        auto plus_i = [&](int r, int c) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] + double(s.Fi[v][r][c]);
+            lam = lam + double(s.Fi[NV][r][c]);
+        };
+        auto minus_i = [&](int r, int c) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] - double(s.Fi[v][r][c]);
+            lam = lam + double(s.Fi[NV][r][c]);
+        };
+        auto plus_j = [&](int r, int c) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] + double(s.Fj[v][r][c]);
+            lam = lam + double(s.Fj[NV][r][c]);
+        };
+        auto minus_j = [&](int r, int c) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] - double(s.Fj[v][r][c]);
+            lam = lam + double(s.Fj[NV][r][c]);
+        };
+        // interior faces (residualCalc.h:199-253)
+        if (i > 0) plus_i(li, lj);
+        if (i < ni - 1) minus_i(li + 1, lj);
+        if (j > 0) plus_j(li, lj);
+        if (j < nj - 1) minus_j(li, lj + 1);
+        // boundaries in id order (residualCalc.h:301-305)
+        if (i == 0) {
+            plus_i(li, lj);   // periodic wrap (+F on cell 0), ghost face (+F), wall (+F)
+        }
+        if (i == ni - 1) {
+            if (a.bc[1] == BC_WALL) plus_i(li + 1, lj);
+            else minus_i(li + 1, lj);   // periodic wrap (-F on cell ni-1), ghost face (-F)
+        }
+        if (j == 0) {
+            plus_j(li, lj);
+        }
+        if (j == nj - 1) {
+            if (a.bc[3] == BC_WALL) plus_j(li, lj + 1);
+            else minus_j(li, lj + 1);
+        }
+        const long long c = (long long)i * nj + j;
+        if (a.mode == MODE_RESIDUAL) {
+            for (int v = 0; v < NV; ++v) a.res_out[c * (NV + 1) + v] = R[v];
+            a.res_out[c * (NV + 1) + NV] = lam;
+            s.flg[tid] = flag;
+            return;
+        }
+        // Runge-Kutta accumulation (rungeKutta.h:176-197)
+        const RkPlan& p = a.plan;
+        double Rt[NV];
+        for (int v = 0; v < NV; ++v) {
+            const double base = p.base >= 0 ? a.acc[((long long)p.base * NV + v) * a.N + c] : 0.0;
+            Rt[v] = base + R[v] * p.a_ss;
+        }
+        for (int u = 0; u < p.nupd; ++u) {
+            for (int v = 0; v < NV; ++v) {
+                double* A = &a.acc[((long long)p.upd_slot[u] * NV + v) * a.N + c];
+                const double old = p.upd_first[u] ? 0.0 : *A;
+                *A = old + R[v] * p.upd_coef[u];
+            }
+        }
+        const double vol = a.m.cvol[c];
+        if (a.mode == MODE_GLOBAL_STAGE0) {
+            for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
+            s.red[tid] = lam / vol;   // spectralRadius.h:399-401
+            s.flg[tid] = flag;
+            return;
+        }
+        double fac;
+        if (a.dt_mode == 0) {
+            const double dt = a.dtslot[0];
+            fac = (p.beta * dt) / vol;
+        } else {
+            double l0;
+            if (a.stage == 0) {
+                l0 = lam;
+                a.lam0[c] = lam;
+            } else {
+                l0 = a.lam0[c];
+            }
+            fac = (p.beta * a.cfl) / l0;
+        }
+        update_cell(a, c, Rt, fac, flag);
+        s.flg[tid] = flag;
+    }
+
+    // q_out = S( C(q0) + R_tot*fac )   (spatial/eulerForwardUpdate.h:274-283)
+    HC_HD static void update_cell(const StageArgs<RF>& a, long long c, const double* Rt, double fac, unsigned int& flag)
+    {
+        double q0[NV], qc[NV], qn[NV];
+        for (int v = 0; v < NV; ++v) q0[v] = a.q0[v * a.N + c];
+        to_conserved<double, LAW, BASIS>(a.sp, q0, qc);
+        for (int v = 0; v < NV; ++v) qc[v] = qc[v] + Rt[v] * fac;
+        from_conserved<double, LAW, BASIS>(a.sp, qc, qn);
+        bool fin = true;
+        for (int v = 0; v < NV; ++v) {
+            a.q_out[v * a.N + c] = qn[v];
+            fin = fin && hc_isfinite(qn[v]);
+        }
+        if (!fin) flag |= ST_NONFINITE;
+        if constexpr (LAW == LAW_EULER && BASIS == BASIS_PRIMITIVE) {
+            if (!(qn[2] > 0.0) || !(qn[3] > 0.0)) flag |= ST_NONPHYSICAL;
+        } else if constexpr (LAW == LAW_EULER) {
+            if (!(qn[2] > 0.0)) flag |= ST_NONPHYSICAL;
+        }
+    }
+};
+
+// deferred update of global-dt stage 0 (dt = cfl / max(lambda/vol), rungeKutta.h:133)
+template <int LAW, int BASIS> struct Stage0Update {
+    static constexpr int NV = LawTraits<LAW>::nv;
+    template <class RF>
+    HC_HD static void run(const StageArgs<RF>& a, long long c, unsigned int& flag)
+    {
+        const double mx = bits_dbl(*a.maxslot);
+        const double dt = a.cfl / mx;
+        if (c == 0) {
+            a.dtslot[0] = dt;
+            a.dtslot[1] = a.dtslot[1] + dt;
+        }
+        if (c >= a.N) return;
+        const double stepfac = a.plan.beta * dt;
+        const double fac = stepfac / a.m.cvol[c];
+        double Rt[NV];
+        for (int v = 0; v < NV; ++v) Rt[v] = a.tmp[v * a.N + c];
+        StageTile<LAW, BASIS, FLUX_CENTRAL, RF, 1, 64>::update_cell(a, c, Rt, fac, flag);
+    }
+};
+
+#if defined(__HIPCC__)
+
+// wave-level reductions for gfx950's 64-lane wavefronts
+__device__ inline double wave_max(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ inline unsigned int wave_or(unsigned int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ>
+__global__ void __launch_bounds__(TI * TJ) k_stage(StageArgs<RF> a, int tiles_j, int ntiles)
+{
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ>;
+    __shared__ typename T::Smem s;
+    const int tid = threadIdx.x;
+    // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
+    // 8 XCDs, so give each XCD a contiguous band of tiles (shared halos stay in one L2)
+    const int b = blockIdx.x;
+    const int t = xcd_tile(b, ntiles);
+    const int ti = t / tiles_j, tj = t % tiles_j;
+    const int i0 = ti * TI, j0 = tj * TJ;
+    unsigned int flag = 0;
+    T::load(a, s, i0, j0, tid);
+    __syncthreads();
+    T::qmetric(a, s, i0, j0, tid);
+    __syncthreads();
+    T::faces(a, s, i0, j0, tid, flag);
+    __syncthreads();
+    T::cells(a, s, i0, j0, tid, flag);
+    // work-group reductions: max(lambda/vol) for global-dt stage 0 and the status bits
+    const double m = wave_max(s.red[tid]);
+    const unsigned int f = wave_or(s.flg[tid]);
+    if ((tid & 63) == 0) {
+        if (a.mode == MODE_GLOBAL_STAGE0) atomicMax(a.maxslot, dbl_bits(m));
+        if (f) atomicOr(a.status, f);
+    }
+    if (a.reset_max && b == 0 && tid == 0) *a.maxslot = dbl_bits(DBL_MIN);
+}
+
+template <int LAW, int BASIS, class RF>
+__global__ void __launch_bounds__(256) k_stage0_update(StageArgs<RF> a)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int flag = 0;
+    Stage0Update<LAW, BASIS>::run(a, c, flag);
+    const unsigned int f = wave_or(flag);
+    if ((threadIdx.x & 63) == 0 && f) atomicOr(a.status, f);
+}
+
+#endif
+
+}   // namespace hc

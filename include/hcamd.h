@@ -1,0 +1,173 @@
+/*
+ * hcamd.h -- C-ABI of the MI355X (gfx950) structured-mesh finite-volume hot path:
+ * spatial residual (MUSCL + limiter + flux functor, LSQ gradients, boundary
+ * conditions) and explicit Runge-Kutta time stepping, executed by hand-written
+ * HIP kernels on one GPU.
+ *
+ * The reference (lacrymose/hyperboliConvergence) exposes this path as a
+ * header-only C++20 template surface; its C++ callers keep that surface
+ * (hyperboliconvergence_amd/cpp/, "hcamd/hcamd.hpp"), and the templates forward
+ * here.  Each entry point names the reference interface it replaces
+ * (paths relative to the reference's cpp/inc/).
+ *
+ * Conventions
+ *  - int status return: 0 ok; HC_ERR_* < 0 on invalid arguments or HIP errors;
+ *    HC_ERR_STATUS when the device raised a status bit (hc_get_status).
+ *  - Host arrays are borrowed for the duration of the call and are row-major
+ *    with the last index fastest, like the reference's par::Array
+ *    (parallalg/array.h:267-349):
+ *      nodes [ni+1][nj+1][2], q [ni][nj][nvar], res [ni][nj][nvar+1],
+ *      bref  [2*nj + 2*ni][nvar] (boundary ids 0,1 along j, then 2,3 along i).
+ *  - One context per host thread; a context is not thread-safe; calls on a
+ *    context are serialised on its HIP stream and are synchronous unless noted.
+ */
+#ifndef HCAMD_H
+#define HCAMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HC_OK 0
+#define HC_ERR_ARG (-1)
+#define HC_ERR_HIP (-2)
+#define HC_ERR_STATE (-3)
+#define HC_ERR_STATUS (-4)
+
+/* identifiers (same numbering as the oracle and the C++ drop-in layer) */
+enum { HC_LAW_SCALAR = 0, HC_LAW_EULER = 1 };
+enum { HC_BASIS_CONSERVED = 0, HC_BASIS_PRIMITIVE = 1 };
+enum { HC_FLUX_CENTRAL = 0, HC_FLUX_RUSANOV = 1, HC_FLUX_ROE = 2, HC_FLUX_SLAU = 3, HC_FLUX_AUSMPUP = 4 };
+enum { HC_SCALE_CONVECTIVE = 0, HC_SCALE_ACOUSTIC = 1, HC_SCALE_ADAPTIVE = 2 };
+enum { HC_LIM_NOLIMIT1 = 0, HC_LIM_NOLIMIT2 = 1, HC_LIM_NOLIMIT3 = 2, HC_LIM_VANALBADA2 = 3,
+       HC_LIM_MINMOD2 = 4, HC_LIM_SUPERBEE2 = 5, HC_LIM_MC2 = 6, HC_LIM_VANLEER2 = 7,
+       HC_LIM_CADA3 = 8, HC_LIM_SMOOTHVANALBADA = 9 };
+enum { HC_BC_PERIODIC = 0, HC_BC_RIEMANN = 1, HC_BC_ENTROPY = 2, HC_BC_FIXED = 3, HC_BC_WALL = 4 };
+enum { HC_DT_GLOBAL = 0, HC_DT_LOCAL = 1 };
+enum { HC_PRECISION_FP64 = 0, HC_PRECISION_MIXED = 1 };
+
+/* status bits (hc_get_status) */
+#define HC_STATUS_NONFINITE 1u
+#define HC_STATUS_NONPHYSICAL 2u
+#define HC_STATUS_BCNORMAL 4u
+
+/* Species<Euler,Real> (conservationLaws/euler/euler.h:48-60) */
+typedef struct {
+    double gamma, minf, lref, nu, pr, dt, R, gamma1;
+} hc_species;
+
+/*
+ * Case descriptor: the runtime form of the reference's template arguments of
+ * make_muscl_flux<Law>(Limiter{}, Flux{}) (spatial/muscl.h:10-60), the boundary
+ * condition tuple (spatial/boundary/boundaryCondition.h:57-136), the Runge-Kutta
+ * table (ode.h:87-105), UnsteadyTimeControls::cfl (controls.h:11-19) and Species.
+ * period_i / period_j: translation from a cell to its periodic image across
+ * boundary 1 / 3 (zero for meshes that close on themselves, such as an O-grid).
+ */
+typedef struct {
+    int law, basis, flux, vscale, pscale, limiter;
+    int bc[4];
+    int nstages, dt_mode;
+    double alpha[6][6];
+    double beta[6];
+    double cfl;
+    double period_i[2];
+    double period_j[2];
+    hc_species sp;
+} hc_case_desc;
+
+typedef struct hc_context hc_context;
+
+/* Create / destroy a context on a HIP device (owns a stream and device buffers). */
+int hc_create(hc_context** ctx, int device);
+int hc_destroy(hc_context* ctx);
+const char* hc_last_error(const hc_context* ctx);
+
+/*
+ * Mesh nodes of a structured 2-D mesh; replaces Mesh<2,Real> construction and
+ * dual() (mesh/mesh.h:19-57, mesh/dual.ipp:121-162).  Geometry is derived on
+ * the next hc_set_case.
+ */
+int hc_set_mesh(hc_context* ctx, int ni, int nj, const double* nodes);
+
+/*
+ * Flux functor, limiter, boundary conditions, Runge-Kutta table, CFL and
+ * species; builds the one-time geometry (cells, faces, LSQ metrics) and
+ * replaces make_muscl_flux (spatial/muscl.h:10-87), xmetrics
+ * (spatial/lsqMetrics.h:108-150) and the BC tuple.
+ */
+int hc_set_case(hc_context* ctx, const hc_case_desc* desc);
+
+/* FP64 throughout (default) or FP32 face arithmetic with FP64 state, RK and metrics. */
+int hc_set_precision(hc_context* ctx, int precision);
+
+/* Boundary reference values, SolutionField::boundary[b](k,1) (solutionField/solutionField.h:56-91). */
+int hc_set_boundary_refs(hc_context* ctx, const double* bref);
+
+/* Solution interior in the solution basis, SolutionField::interior. */
+int hc_set_state(hc_context* ctx, const double* q);
+int hc_get_state(hc_context* ctx, double* q);
+
+/*
+ * Spatial residual of the current state, FluxResult per cell {flux[nvar], lambda}:
+ * boundaryUpdate + qmetrics + residualCalc (spatial/boundary/boundaryUpdate.h:28-48,
+ * spatial/lsqMetrics.h:269-301, spatial/residualCalc.h:45-69).
+ */
+int hc_residual(hc_context* ctx, double* res);
+
+/*
+ * Advance nsteps time steps: the body of integrate() (timestepping/rungeKutta.h:44-167)
+ * with rungeKuttaAccumulation (:176-197), spectralRadius (spatial/spectralRadius.h:24-48)
+ * and eulerForwardUpdateGlobal / Local (spatial/eulerForwardUpdate.h:236-366).
+ * Asynchronous with respect to the host (hc_synchronize or any getter waits).
+ */
+int hc_step(hc_context* ctx, int nsteps);
+
+/* Sum of the global time steps taken (the reference's t, rungeKutta.h:96,163). */
+int hc_get_time(hc_context* ctx, double* t);
+
+/*
+ * L2 (root mean square) and Linf norms per variable of R/vol of the current
+ * state's residual, accumulated in FP64.  No counterpart at the reference's
+ * HEAD (OLD.fieldOperations/fieldOperations.h:38-42 declares a residual()).
+ */
+int hc_norms(hc_context* ctx, double* l2, double* linf);
+
+/* OR of the status bits raised by the device since the last call (cleared on read). */
+int hc_get_status(hc_context* ctx, unsigned int* status);
+
+int hc_synchronize(hc_context* ctx);
+
+/*
+ * Capture the kernels of k consecutive steps in a HIP graph and replay it in
+ * hc_step (k = 0 disables graphs).  Replaces nothing in the reference (its
+ * integrate() loop runs on the host).
+ */
+int hc_set_graph_steps(hc_context* ctx, int k);
+
+/*
+ * Kernel timing: when enabled, hc_step records HIP events around each launch;
+ * hc_get_profile returns {stage kernel ms, stage-kernel launches, update kernel ms,
+ * update launches}.  Used for the roofline report.
+ */
+int hc_set_profiling(hc_context* ctx, int enable);
+int hc_get_profile(hc_context* ctx, double* out4);
+
+/* Device pointer of the current state ([nvar][ni][nj] FP64, valid until the next call). */
+int hc_device_state(hc_context* ctx, double** dptr);
+
+/*
+ * STREAM triad a[i] = b[i] + s*c[i] over three device arrays of n doubles,
+ * repeated reps times; returns the attainable HBM bandwidth in GB/s (24 n bytes
+ * per repetition).  Used as the attainable-bandwidth denominator and to
+ * calibrate memory counters.  No counterpart in the reference.
+ */
+int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps);
+
+/* Library version string. */
+const char* hc_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,61 @@
+"""ctypes binding of the test-only host emulation of the stage kernels."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        path = os.path.join(HERE, "libhostemu.so")
+        subprocess.run(["make", "-C", HERE, "-s"], check=True, capture_output=True)
+        _L = C.CDLL(path)
+        P, i = C.c_void_p, C.c_int
+        _L.emu_residual.argtypes = [P, i, i, P, P, P, i, P, P]
+        _L.emu_integrate.argtypes = [P, i, i, P, P, P, i, i, P, P]
+        _L.emu_geometry.argtypes = [P, i, i, P, P, P, P, P]
+    return _L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def residual(cs, nodes, q, bref, precision=0):
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    nv = 4 if cs.law == 1 else 3
+    res = np.zeros((ni, nj, nv + 1))
+    st = C.c_uint()
+    rc = lib().emu_residual(C.cast(C.pointer(cs), C.c_void_p), ni, nj, _p(np.ascontiguousarray(nodes)),
+                            _p(np.ascontiguousarray(q, float)), _p(np.ascontiguousarray(bref, float)), precision,
+                            _p(res), C.byref(st))
+    assert rc == 0
+    return res, st.value
+
+
+def integrate(cs, nodes, q, bref, nsteps, precision=0):
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    qq = np.ascontiguousarray(q, float).copy()
+    t = C.c_double()
+    st = C.c_uint()
+    rc = lib().emu_integrate(C.cast(C.pointer(cs), C.c_void_p), ni, nj, _p(np.ascontiguousarray(nodes)), _p(qq),
+                             _p(np.ascontiguousarray(bref, float)), precision, nsteps, C.byref(t), C.byref(st))
+    assert rc == 0
+    return qq, t.value, st.value
+
+
+def geometry(cs, nodes):
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    cells = np.zeros((ni, nj, 3))
+    X = np.zeros((ni, nj, 3))
+    fi = np.zeros((14, (ni + 1) * nj))
+    fj = np.zeros((14, ni * (nj + 1)))
+    rc = lib().emu_geometry(C.cast(C.pointer(cs), C.c_void_p), ni, nj, _p(np.ascontiguousarray(nodes)), _p(cells),
+                            _p(X), _p(fi), _p(fj))
+    assert rc == 0
+    return cells, X, fi, fj
