@@ -29,12 +29,14 @@ def rel_err_conserved(q, qref, law, basis):
     from hyperboliconvergence_amd import cases
     if law == 1 and basis == 1:
         q, qref = cases.prim_to_cons(q), cases.prim_to_cons(qref)
+    if not (np.isfinite(q).all() and np.isfinite(qref).all()):
+        return float("nan")
     errs = []
     for v in range(q.shape[-1]):
         s = np.max(np.abs(qref[..., v]))
         d = np.max(np.abs(q[..., v] - qref[..., v]))
         errs.append(d / s if s > 0 else d)
-    return float(max(errs))
+    return float(np.max(errs))
 
 
 def make_cfg(name, steps):
@@ -51,6 +53,36 @@ def make_cfg(name, steps):
         g = 16.0
     cfg["geom_bytes"] = g
     return cfg
+
+
+def run_state(cfg, precision, steps):
+    """State after `steps` steps from the initial condition (no timing)."""
+    import hyperboliconvergence_amd as hca
+    with hca.Context(0) as ctx:
+        ctx.set_mesh(cfg["nodes"])
+        ctx.set_case(cfg["case"])
+        ctx.set_precision(precision)
+        ctx.set_boundary_refs(cfg["bref"])
+        ctx.set_state(cfg["q"])
+        ctx.step(steps)
+        return ctx.get_state(), ctx.status()
+
+
+def first_nonfinite_step(cfg, steps, chunk=50):
+    """First multiple of `chunk` steps after which the FP64 state holds a non-finite value (None if never)."""
+    import hyperboliconvergence_amd as hca
+    with hca.Context(0) as ctx:
+        ctx.set_mesh(cfg["nodes"])
+        ctx.set_case(cfg["case"])
+        ctx.set_boundary_refs(cfg["bref"])
+        ctx.set_state(cfg["q"])
+        done = 0
+        while done < steps:
+            ctx.step(chunk)
+            done += chunk
+            if ctx.status() & 1:
+                return done
+    return None
 
 
 def run_gpu(cfg, precision, steps, warmup, graph_steps):
@@ -73,6 +105,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     t1 = time.perf_counter()
     q = ctx.get_state()
     st = ctx.status()
+    ctx.status()   # clear
     # live kernel timing with HIP events on the context's stream (no graphs while profiling)
     ctx.set_state(cfg["q"])
     ctx.set_profiling(True)
@@ -93,9 +126,8 @@ def cpu_baseline(cfg, budget_s=12.0):
     cs = cfg["case"]
     ni, nj = cfg["nodes"].shape[0] - 1, cfg["nodes"].shape[1] - 1
     ns = cs.nstages
-    t0 = time.perf_counter()
-    po.integrate(cs, cfg["nodes"], cfg["q"], 1, cfg["bref"], fast=True)
-    per_step = max(time.perf_counter() - t0, 1e-6)
+    _, _, tm1 = po.integrate(cs, cfg["nodes"], cfg["q"], 1, cfg["bref"], fast=True)
+    per_step = max(tm1.sum() / 1e3, 1e-6)
     nsteps = int(max(1, min(cfg["nsteps"], budget_s / per_step)))
     best = None
     t0 = time.perf_counter()
@@ -159,12 +191,39 @@ def main():
               f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}",
               file=sys.stderr, flush=True)
 
+    # the reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:84-91) and the cylinder case
+    # diverges after a few hundred steps in the reference algorithm itself: the mixed-vs-FP64 error is
+    # measured over the window before the FP64 state leaves the finite numbers
+    diverge = first_nonfinite_step(cfg, steps) if a.workload == "c2" else None
+    err_steps = steps if diverge is None else max(50, diverge - 100)
     mixed_err = None
     if 0 in res and 1 in res:
-        mixed_err = rel_err_conserved(res[1]["q"], res[0]["q"], cs.law, cs.basis)
-        print(f"# mixed vs fp64 state error after {steps} steps: {mixed_err:.3e}", file=sys.stderr, flush=True)
-    head = 1 if (1 in res and (mixed_err is None or mixed_err <= 1e-6)) else 0
+        q64, _ = run_state(cfg, 0, err_steps)
+        q32, _ = run_state(cfg, 1, err_steps)
+        mixed_err = rel_err_conserved(q32, q64, cs.law, cs.basis)
+        print(f"# mixed vs fp64 state error after {err_steps} steps: {mixed_err:.3e} "
+              f"(fp64 first non-finite at step {diverge})", file=sys.stderr, flush=True)
+    head = 1 if (1 in res and mixed_err is not None and mixed_err <= 1e-6) else 0
     h = res[head]
+
+    # the same workload with the face-frame Roe variant (stable on the O-grid)
+    ff = None
+    if a.workload == "c2" and a.precision == "both":
+        import hyperboliconvergence_amd as hca
+        cfg_ff = dict(cfg)
+        cfg_ff["case"] = type(cs).from_buffer_copy(cs)
+        cfg_ff["case"].flux = hca.FLUX_ROE_FACEFRAME
+        ff = {}
+        for prec in (0, 1):
+            el, q, st, prof = run_gpu(cfg_ff, prec, steps, a.warmup, a.graph_steps)
+            ff[prec] = dict(rate=N * ns * steps / el, q=q, status=st, kernel_us=prof[0] / max(prof[1], 1.0) * 1e3)
+        ff_err = rel_err_conserved(ff[1]["q"], ff[0]["q"], cs.law, cs.basis)
+        print(f"# face-frame Roe: fp64 {ff[0]['rate']:.4e}, mixed {ff[1]['rate']:.4e} cell-stage/s, "
+              f"mixed vs fp64 after {steps} steps {ff_err:.3e}, status {ff[0]['status']}/{ff[1]['status']}",
+              file=sys.stderr, flush=True)
+        ff = {"flux": "RoeFluxFaceFrame", "fp64_value": ff[0]["rate"], "mixed_value": ff[1]["rate"],
+              "fp64_kernel_us": ff[0]["kernel_us"], "mixed_kernel_us": ff[1]["kernel_us"],
+              "mixed_vs_fp64_rel_err": ff_err, "status": [int(ff[0]["status"]), int(ff[1]["status"])]}
 
     import hyperboliconvergence_amd as hca
     with hca.Context(0) as c:
@@ -202,7 +261,10 @@ def main():
         "fp64_value": res[0]["rate"] if 0 in res else None,
         "mixed_value": res[1]["rate"] if 1 in res else None,
         "mixed_vs_fp64_rel_err": mixed_err,
+        "mixed_err_steps": err_steps,
+        "fp64_first_nonfinite_step": diverge,
         "status": int(h["status"]),
+        "face_frame_roe": ff,
     }
     print(json.dumps(out), flush=True)
 

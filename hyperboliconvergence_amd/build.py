@@ -25,20 +25,21 @@ ARCH = "gfx950"
 COMMON = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-I", CSRC, "-I", INCLUDE,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
-KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(5)]
+KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(6)]
 
 
-def _jobs():
+def _jobs(mixed_contract="fast", build_dir=None):
     objs = []
+    bdir = build_dir or BUILD
     for (law, basis, flux) in KERNELS:
         for prec in (64, 32):
-            contract = "-ffp-contract=off" if prec == 64 else "-ffp-contract=fast"
-            out = os.path.join(BUILD, f"stage_L{law}_B{basis}_F{flux}_P{prec}.o")
+            contract = "-ffp-contract=off" if prec == 64 else f"-ffp-contract={mixed_contract}"
+            out = os.path.join(bdir, f"stage_L{law}_B{basis}_F{flux}_P{prec}.o")
             cmd = [HIPCC, *COMMON, contract, f"-DHC_LAW={law}", f"-DHC_BASIS={basis}", f"-DHC_FLUX={flux}",
                    f"-DHC_PREC={prec}", "-x", "hip", "-c", os.path.join(CSRC, "hc_kernel_inst.hip"), "-o", out]
             objs.append((out, cmd))
     for src in ("hc_geom.cpp", "hc_capi.cpp"):
-        out = os.path.join(BUILD, src.replace(".cpp", ".o"))
+        out = os.path.join(bdir, src.replace(".cpp", ".o"))
         cmd = [HIPCC, *COMMON, "-ffp-contract=off", "-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
         objs.append((out, cmd))
     return objs
@@ -52,27 +53,31 @@ def _run(job):
     return out
 
 
-def build(jobs=None, verbose=False):
+def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=None):
     """Compile every HIP source and link libhcamd.so; always recompiles."""
-    os.makedirs(BUILD, exist_ok=True)
-    work = _jobs()
+    bdir = build_dir or BUILD
+    os.makedirs(bdir, exist_ok=True)
+    work = _jobs(mixed_contract, bdir)
     n = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=n) as ex:
         objs = list(ex.map(_run, work))
-    tmp = LIB + ".tmp"
+    tmp = lib + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, lib)
     if verbose:
-        print(f"built {LIB} from {len(objs)} objects")
-    return LIB
+        print(f"built {lib} from {len(objs)} objects")
+    return lib
 
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--out", default=LIB)
+    ap.add_argument("--mixed-contract", default="fast", choices=["fast", "on", "off"])
     a = ap.parse_args()
-    build(a.jobs, verbose=True)
+    build(a.jobs, verbose=True, lib=a.out, mixed_contract=a.mixed_contract,
+          build_dir=None if a.out == LIB else os.path.join(BUILD, "alt"))
     sys.exit(0)

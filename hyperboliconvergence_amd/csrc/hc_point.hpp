@@ -24,7 +24,7 @@ namespace hc {
 
 enum Law { LAW_SCALAR = 0, LAW_EULER = 1 };
 enum Basis { BASIS_CONSERVED = 0, BASIS_PRIMITIVE = 1 };
-enum FluxId { FLUX_CENTRAL = 0, FLUX_RUSANOV = 1, FLUX_ROE = 2, FLUX_SLAU = 3, FLUX_AUSMPUP = 4 };
+enum FluxId { FLUX_CENTRAL = 0, FLUX_RUSANOV = 1, FLUX_ROE = 2, FLUX_SLAU = 3, FLUX_AUSMPUP = 4, FLUX_ROE_FACEFRAME = 5 };
 enum Scale { SCALE_CONVECTIVE = 0, SCALE_ACOUSTIC = 1, SCALE_ADAPTIVE = 2 };
 enum LimiterId {
     LIM_NOLIMIT1 = 0, LIM_NOLIMIT2 = 1, LIM_NOLIMIT3 = 2, LIM_VANALBADA2 = 3, LIM_MINMOD2 = 4,
@@ -460,11 +460,22 @@ template <class R> HC_HD void entropyfix(const R* la, const R* ll, const R* lr, 
     }
 }
 
-// base/fluxes/roeFlux.ipp:39-95 (RoeFlux = CentralFlux + RoeDissipation)
-template <class R> HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
+// base/fluxes/roeFlux.ipp:39-95 (RoeFlux = CentralFlux + RoeDissipation).  FACEFRAME = true is the
+// added variant that rotates the Roe-average velocity into the face frame like the jump; the
+// reference keeps it in the global frame (roeFlux.ipp:84-91).
+template <class R, bool FACEFRAME = false>
+HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
 {
     const Flux4<R> c = euler_central(f, sl, sr);
     const EState<R> ravg = roe_average(sp, sl, sr);
+    EState<R> rv = ravg;
+    if constexpr (FACEFRAME) {
+        const R uv[2] = {ravg.u, ravg.v};
+        R a[2];
+        rotate_to_face(f, uv, a);
+        rv.u = a[0];
+        rv.v = a[1];
+    }
     R wa[4], wl[4], wr[4], lam[4];
     wavespeeds(f, ravg, wa);
     wavespeeds(f, sl, wl);
@@ -478,12 +489,12 @@ template <class R> HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>
     dqc[2] = dq[2];
     dqc[3] = dq[3];
     R dqp[4], dqw[4];
-    dcons_to_dprim(sp, ravg, dqc, dqp);
-    dprim_to_dchar(sp, ravg, dqp, dqw);
+    dcons_to_dprim(sp, rv, dqc, dqp);
+    dprim_to_dchar(sp, rv, dqp, dqw);
     for (int v = 0; v < 4; ++v) dqw[v] = lam[v] * dqw[v];
     R dqp2[4], dqc2[4], fd[4];
-    dchar_to_dprim(sp, ravg, dqw, dqp2);
-    dprim_to_dcons(sp, ravg, dqp2, dqc2);
+    dchar_to_dprim(sp, rv, dqw, dqp2);
+    dprim_to_dcons(sp, rv, dqp2, dqc2);
     rotate_from_face(f, dqc2, fd);
     fd[2] = dqc2[2];
     fd[3] = dqc2[3];
@@ -631,6 +642,7 @@ HC_HD Flux4<R> face_flux(const Species<R>& sp, int vscale, int pscale, const Fac
         else if constexpr (FLUX == FLUX_RUSANOV) return euler_rusanov(f, sl, sr);
         else if constexpr (FLUX == FLUX_ROE) return euler_roe(sp, f, sl, sr);
         else if constexpr (FLUX == FLUX_SLAU) return euler_slau(sp, f, sl, sr);
+        else if constexpr (FLUX == FLUX_ROE_FACEFRAME) return euler_roe<R, true>(sp, f, sl, sr);
         else return euler_ausmpup(sp, vscale, pscale, f, sl, sr);
     }
 }

@@ -529,11 +529,23 @@ void entropyfix(const double* la, const double* ll, const double* lr, double* s)
     }
 }
 
-// base/fluxes/roeFlux.ipp:59-95
-FluxRes roe_dissipation(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr)
+// base/fluxes/roeFlux.ipp:59-95.  face_frame = true is the added variant whose Roe-average
+// velocity is rotated into the face frame like the jump (the reference keeps it in the global frame).
+FluxRes roe_dissipation(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr,
+                        bool face_frame = false)
 {
     double ravg[8];
     roe_average(sp, sl, sr, ravg);
+    double rot[8];
+    for (int k = 0; k < 8; ++k) rot[k] = ravg[k];
+    if (face_frame) {
+        const double uv[2] = {ravg[SU], ravg[SVV]};
+        double a[2];
+        rotate_to(f, uv, a, 2);
+        rot[SU] = a[0];
+        rot[SVV] = a[1];
+    }
+    const double* rv = rot;
     double wa[4], wl[4], wr[4], lambda[4];
     wavespeeds(f.n, ravg, wa);
     wavespeeds(f.n, sl, wl);
@@ -545,12 +557,12 @@ FluxRes roe_dissipation(const orc_case& cs, const Species& sp, const Face& f, co
     for (int v = 0; v < 4; ++v) dq[v] = cr[v] - cl[v];
     rotate_to(f, dq, dqc, 4);
     double dqp[4], dqw[4];
-    d_cons2prim(sp, ravg, dqc, dqp);
-    d_prim2char(sp, ravg, dqp, dqw);
+    d_cons2prim(sp, rv, dqc, dqp);
+    d_prim2char(sp, rv, dqp, dqw);
     for (int v = 0; v < 4; ++v) dqw[v] = lambda[v] * dqw[v];
     double dqp2[4], dqc2[4], fd[4];
-    d_char2prim(sp, ravg, dqw, dqp2);
-    d_prim2cons(sp, ravg, dqp2, dqc2);
+    d_char2prim(sp, rv, dqw, dqp2);
+    d_prim2cons(sp, rv, dqp2, dqc2);
     rotate_from(f, dqc2, fd, 4);
     FluxRes out{};
     for (int v = 0; v < 4; ++v) out.f[v] = fd[v] * (-0.5 * f.area);
@@ -559,10 +571,11 @@ FluxRes roe_dissipation(const orc_case& cs, const Species& sp, const Face& f, co
 }
 
 // base/fluxes/roeFlux.ipp:39-56
-FluxRes roe_flux(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr)
+FluxRes roe_flux(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr,
+                 bool face_frame = false)
 {
     const FluxRes c = central_flux(cs, f, sl, sr);
-    const FluxRes d = roe_dissipation(cs, sp, f, sl, sr);
+    const FluxRes d = roe_dissipation(cs, sp, f, sl, sr, face_frame);
     FluxRes out{};
     for (int v = 0; v < 4; ++v) out.f[v] = c.f[v] + d.f[v];
     out.lambda = std::fmax(c.lambda, d.lambda);
@@ -694,6 +707,7 @@ FluxRes flux_states(const orc_case& cs, const Species& sp, const Face& f, const 
     case ORC_FLUX_RUSANOV: return rusanov_flux(cs, f, sl, sr);
     case ORC_FLUX_ROE: return roe_flux(cs, sp, f, sl, sr);
     case ORC_FLUX_SLAU: return slau_flux(cs, sp, f, sl, sr);
+    case ORC_FLUX_ROE_FACEFRAME: return roe_flux(cs, sp, f, sl, sr, true);
     default: return ausm_flux(cs, sp, f, sl, sr);
     }
 }
@@ -1260,6 +1274,7 @@ bool valid_case(const orc_case& cs, int ni, int nj)
     if ((cs.bc[0] == ORC_BC_PERIODIC) != (cs.bc[1] == ORC_BC_PERIODIC)) return false;
     if ((cs.bc[2] == ORC_BC_PERIODIC) != (cs.bc[3] == ORC_BC_PERIODIC)) return false;
     if (cs.law == ORC_LAW_SCALAR && cs.flux > ORC_FLUX_RUSANOV) return false;
+    if (cs.flux < 0 || cs.flux > ORC_FLUX_ROE_FACEFRAME) return false;
     if (cs.nstages < 1 || cs.nstages > 6) return false;
     return true;
 }

@@ -1,0 +1,62 @@
+"""The C++20 drop-in layer (hyperboliconvergence_amd/cpp/hcamd/hcamd.hpp): its point functors
+evaluated on the host against the oracle, and the example driver built against libhcamd.so."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CPP = os.path.join(ROOT, "hyperboliconvergence_amd", "cpp")
+
+
+def _build_points():
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_points")
+    subprocess.run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-I", CPP, "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "cpp", "dropin_points.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def test_dropin_point_functors_bit_identical_to_oracle():
+    exe = _build_points()
+    rng = np.random.default_rng(21)
+    lines, cases = [], []
+    for k in range(60):
+        fid = k % 5
+        lid = (4, 3, 7)[k % 3]
+        n = rng.standard_normal(2)
+        n /= np.linalg.norm(n)
+        area = rng.uniform(0.1, 2.0)
+        ql = np.array([*rng.uniform(-0.5, 0.5, 2), rng.uniform(0.5, 1.5), rng.uniform(0.5, 1.0)])
+        qr = np.array([*rng.uniform(-0.5, 0.5, 2), rng.uniform(0.5, 1.5), rng.uniform(0.5, 1.0)])
+        lines.append(" ".join(repr(float(x)) for x in [fid, lid, *n, area, *ql, *qr]).replace(".0 ", " ", 0))
+        cases.append((fid, lid, n, area, ql, qr))
+    text = "\n".join(f"{c[0]} {c[1]} " + " ".join(repr(float(x)) for x in [*c[2], c[3], *c[4], *c[5]]) for c in cases)
+    out = subprocess.run([exe], input=text, capture_output=True, text=True, check=True).stdout.split("\n")
+    for k, (fid, lid, n, area, ql, qr) in enumerate(cases):
+        got = np.array(list(map(float, out[2 * k].split())))
+        cs = po.make_case(law=po.LAW_EULER, flux=fid, minf=0.3)
+        f = po.face_from(area, (0.0, 0.0), n)
+        assert np.array_equal(got, po.flux(cs, f, ql, qr)), (k, fid)
+        lim = np.array(list(map(float, out[2 * k + 1].split())))
+        assert np.array_equal(lim, [po.limit(lid, a, b) for a, b in zip(ql, qr)])
+
+
+def test_example_driver_builds():
+    subprocess.run(["make", "-C", CPP, "-B", "examples/cylinder2D"], check=True, capture_output=True)
+    assert os.path.exists(os.path.join(CPP, "examples", "cylinder2D"))
+
+
+@pytest.mark.gpu
+def test_example_driver_runs_and_matches_oracle():
+    from hyperboliconvergence_amd import cases
+    exe = os.path.join(CPP, "examples", "cylinder2D")
+    assert os.path.exists(exe), "build it with make -C hyperboliconvergence_amd/cpp"
+    out = subprocess.run([exe, "64", "32", "50"], capture_output=True, text=True, check=True, timeout=120).stdout
+    checksum = float(out.split("checksum")[1].split()[0])
+    # the C++ O-grid generator uses the same formula as cases.mesh_ogrid
+    cfg = cases.config_c2(nth=64, nr=32)
+    qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 50, cfg["bref"])
+    assert abs(checksum - qo.sum()) <= 1e-9 * abs(qo.sum())
