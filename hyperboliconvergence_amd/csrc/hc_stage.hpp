@@ -139,7 +139,9 @@ HC_HD double bits_dbl(unsigned long long u)
     return x;
 }
 
-HC_HD bool hc_isfinite(double x) { return x - x == 0.0; }
+// exponent-bit test: immune to floating-point contraction, which may turn x - x with x = a*b
+// into fma(a, b, -a*b), the (non-zero) rounding error of the product
+HC_HD bool hc_isfinite(double x) { return (dbl_bits(x) & 0x7FF0000000000000ULL) != 0x7FF0000000000000ULL; }
 
 // Bijection of work-group ids onto tiles such that the ids an XCD receives
 // (b % 8 == x) cover one contiguous band of tiles.

@@ -150,6 +150,8 @@ def test_mixed_precision_close_to_fp64_oracle():
         r = ctx.residual()
         ctx.step(50)
         q = ctx.get_state()
+        # regression: the finiteness test must not be fooled by FP contraction in the mixed objects
+        assert ctx.status() == 0
     ro = _oracle_res(cfg)
     qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 50, cfg["bref"])
     # FP32 face fluxes carry round-off of eps_32 * |F|, and |F| >> |R| near a balanced
