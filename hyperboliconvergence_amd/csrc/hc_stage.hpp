@@ -160,12 +160,12 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTi
     static constexpr int MH = TI + 2, MW = TJ + 2;   // Q ring with 1-cell halo
     static constexpr int NFI = (TI + 1) * TJ;
     static constexpr int NFJ = TI * (TJ + 1);
+    static constexpr int NF = NFI + NFJ;
 
     struct Smem {
         RF q[NV][QH][QW];
         RF Q[NV][2][MH][MW];
-        RF Fi[NV + 1][TI + 1][TJ];
-        RF Fj[NV + 1][TI][TJ + 1];
+        RF F[NV + 1][NF];   // face fluxes and spectral radii, i-faces then j-faces
         double red[NT];
         unsigned int flg[NT];
     };
@@ -301,76 +301,62 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTi
         return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, fres, qint, qg);
     }
 
-    HC_HD static void store_i(Smem& s, int li, int lj, const Flux4<RF>& F)
+    HC_HD static void store(Smem& s, int k, const Flux4<RF>& F)
     {
-        for (int v = 0; v < NV; ++v) s.Fi[v][li][lj] = F.f[v];
-        s.Fi[NV][li][lj] = F.lambda;
-    }
-    HC_HD static void store_j(Smem& s, int li, int lj, const Flux4<RF>& F)
-    {
-        for (int v = 0; v < NV; ++v) s.Fj[v][li][lj] = F.f[v];
-        s.Fj[NV][li][lj] = F.lambda;
+        for (int v = 0; v < NV; ++v) s.F[v][k] = F.f[v];
+        s.F[NV][k] = F.lambda;
     }
 
     // ---- phase 3: all faces of the tile ----
+    // One face index space: i-faces k = li*TJ + lj (node line i0+li, row j0+lj), then j-faces
+    // k = NFI + li*(TJ+1) + lj (row i0+li, node line j0+lj).  Both kinds run the same MUSCL and
+    // flux code on lane-selected operands, so a wave holding both does not diverge.
     HC_HD static void faces(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
     {
         const int ni = a.m.ni, nj = a.m.nj;
-        for (int k = tid; k < NFI + NFJ; k += NT) {
-            RF ql[NV], qr[NV], Ql[NV][2], Qr[NV][2];
-            if (k < NFI) {
-                // i-face on node line l between cells (l-1, j) and (l, j)
-                const int li = k / TJ, lj = k % TJ;
-                const int l = i0 + li, j = j0 + lj;
-                if (l > ni || j >= nj) continue;
-                const bool interior = l >= 1 && l <= ni - 1;
-                if (interior || a.m.per_i) {
-                    const long long fidx = (long long)(interior ? l : 0) * nj + j;
-                    for (int v = 0; v < NV; ++v) {
-                        ql[v] = s.q[v][li + 1][lj + 2];
-                        qr[v] = s.q[v][li + 2][lj + 2];
-                        Ql[v][0] = s.Q[v][0][li][lj + 1];
-                        Ql[v][1] = s.Q[v][1][li][lj + 1];
-                        Qr[v][0] = s.Q[v][0][li + 1][lj + 1];
-                        Qr[v][1] = s.Q[v][1][li + 1][lj + 1];
-                    }
-                    store_i(s, li, lj, muscl(a, a.m.fi, a.m.nfi, fidx, ql, qr, Ql, Qr));
-                } else {
-                    const int bid = (l == 0) ? 0 : 1;
-                    const int ic = (l == 0) ? 0 : ni - 1;
-                    const int lr = (l == 0) ? li + 2 : li + 1;
-                    for (int v = 0; v < NV; ++v) ql[v] = s.q[v][lr][lj + 2];
-                    const long long fidx = (long long)l * nj + j;
-                    const long long row = (bid == 0 ? 0 : nj) + j;
-                    store_i(s, li, lj, boundary_face(a, bid, a.m.fi, a.m.nfi, fidx, ql, (long long)ic * nj + j, row, flag));
+        const RF* qs = &s.q[0][0][0];
+        const RF* Qs = &s.Q[0][0][0][0];
+        for (int k = tid; k < NF; k += NT) {
+            const bool isi = k < NFI;
+            const int kk = isi ? k : k - NFI;
+            const int w = isi ? TJ : TJ + 1;
+            const int li = kk / w, lj = kk % w;
+            const int l = isi ? i0 + li : j0 + lj;        // node line of the face
+            const int t = isi ? j0 + lj : i0 + li;        // transverse cell index
+            const int lmax = isi ? ni : nj;
+            if (l > lmax || t >= (isi ? nj : ni)) continue;
+            const bool interior = l >= 1 && l <= lmax - 1;
+            const RF* base = isi ? a.m.fi : a.m.fj;
+            const long long nf = isi ? a.m.nfi : a.m.nfj;
+            const int pr = (li + 2) * QW + lj + 2;        // right cell in the q tile
+            const int pl = pr - (isi ? QW : 1);            // left cell
+            RF ql[NV], qr[NV];
+            if (interior || (isi ? a.m.per_i : a.m.per_j)) {
+                const int le = interior ? l : 0;
+                const long long fidx = isi ? (long long)le * nj + t : (long long)t * (nj + 1) + le;
+                const int mr = (li + 1) * MW + lj + 1;     // right cell in the Q ring
+                const int ml = mr - (isi ? MW : 1);
+                RF Ql[NV][2], Qr[NV][2];
+                for (int v = 0; v < NV; ++v) {
+                    ql[v] = qs[v * QH * QW + pl];
+                    qr[v] = qs[v * QH * QW + pr];
+                    Ql[v][0] = Qs[(2 * v) * MH * MW + ml];
+                    Ql[v][1] = Qs[(2 * v + 1) * MH * MW + ml];
+                    Qr[v][0] = Qs[(2 * v) * MH * MW + mr];
+                    Qr[v][1] = Qs[(2 * v + 1) * MH * MW + mr];
                 }
+                store(s, k, muscl(a, base, nf, fidx, ql, qr, Ql, Qr));
             } else {
-                // j-face on node line l between cells (i, l-1) and (i, l)
-                const int kk = k - NFI;
-                const int li = kk / (TJ + 1), lj = kk % (TJ + 1);
-                const int i = i0 + li, l = j0 + lj;
-                if (i >= ni || l > nj) continue;
-                const bool interior = l >= 1 && l <= nj - 1;
-                if (interior || a.m.per_j) {
-                    const long long fidx = (long long)i * (nj + 1) + (interior ? l : 0);
-                    for (int v = 0; v < NV; ++v) {
-                        ql[v] = s.q[v][li + 2][lj + 1];
-                        qr[v] = s.q[v][li + 2][lj + 2];
-                        Ql[v][0] = s.Q[v][0][li + 1][lj];
-                        Ql[v][1] = s.Q[v][1][li + 1][lj];
-                        Qr[v][0] = s.Q[v][0][li + 1][lj + 1];
-                        Qr[v][1] = s.Q[v][1][li + 1][lj + 1];
-                    }
-                    store_j(s, li, lj, muscl(a, a.m.fj, a.m.nfj, fidx, ql, qr, Ql, Qr));
-                } else {
-                    const int bid = (l == 0) ? 2 : 3;
-                    const int jc = (l == 0) ? 0 : nj - 1;
-                    const int lc = (l == 0) ? lj + 2 : lj + 1;
-                    for (int v = 0; v < NV; ++v) ql[v] = s.q[v][li + 2][lc];
-                    const long long fidx = (long long)i * (nj + 1) + l;
-                    const long long row = 2LL * nj + (bid == 2 ? 0 : ni) + i;
-                    store_j(s, li, lj, boundary_face(a, bid, a.m.fj, a.m.nfj, fidx, ql, (long long)i * nj + jc, row, flag));
-                }
+                // the interior cell is right of node line 0 and left of the last node line
+                const bool first = l == 0;
+                const int bid = (isi ? 0 : 2) + (first ? 0 : 1);
+                const int nc = first ? 0 : lmax - 1;
+                const int pc = first ? pr : pl;
+                for (int v = 0; v < NV; ++v) ql[v] = qs[v * QH * QW + pc];
+                const long long fidx = isi ? (long long)l * nj + t : (long long)t * (nj + 1) + l;
+                const long long cint = isi ? (long long)nc * nj + t : (long long)t * nj + nc;
+                const long long row = isi ? (first ? 0 : nj) + t : 2LL * nj + (first ? 0 : ni) + t;
+                store(s, k, boundary_face(a, bid, base, nf, fidx, ql, cint, row, flag));
             }
         }
     }
@@ -389,41 +375,35 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTi
         double R[NV];
         double lam = 0.0;
         for (int v = 0; v < NV; ++v) R[v] = 0.0;
-        auto plus_i = [&](int r, int c) {
-            for (int v = 0; v < NV; ++v) R[v] = R[v] + double(s.Fi[v][r][c]);
-            lam = lam + double(s.Fi[NV][r][c]);
+        auto plus = [&](int k) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] + double(s.F[v][k]);
+            lam = lam + double(s.F[NV][k]);
         };
-        auto minus_i = [&](int r, int c) {
-            for (int v = 0; v < NV; ++v) R[v] = R[v] - double(s.Fi[v][r][c]);
-            lam = lam + double(s.Fi[NV][r][c]);
+        auto minus = [&](int k) {
+            for (int v = 0; v < NV; ++v) R[v] = R[v] - double(s.F[v][k]);
+            lam = lam + double(s.F[NV][k]);
         };
-        auto plus_j = [&](int r, int c) {
-            for (int v = 0; v < NV; ++v) R[v] = R[v] + double(s.Fj[v][r][c]);
-            lam = lam + double(s.Fj[NV][r][c]);
-        };
-        auto minus_j = [&](int r, int c) {
-            for (int v = 0; v < NV; ++v) R[v] = R[v] - double(s.Fj[v][r][c]);
-            lam = lam + double(s.Fj[NV][r][c]);
-        };
+        const int kil = li * TJ + lj, kir = kil + TJ;                   // i-faces left / right
+        const int kjl = NFI + li * (TJ + 1) + lj, kjr = kjl + 1;       // j-faces left / right
         // interior faces (residualCalc.h:199-253)
-        if (i > 0) plus_i(li, lj);
-        if (i < ni - 1) minus_i(li + 1, lj);
-        if (j > 0) plus_j(li, lj);
-        if (j < nj - 1) minus_j(li, lj + 1);
+        if (i > 0) plus(kil);
+        if (i < ni - 1) minus(kir);
+        if (j > 0) plus(kjl);
+        if (j < nj - 1) minus(kjr);
         // boundaries in id order (residualCalc.h:301-305)
         if (i == 0) {
-            plus_i(li, lj);   // periodic wrap (+F on cell 0), ghost face (+F), wall (+F)
+            plus(kil);   // periodic wrap (+F on cell 0), ghost face (+F), wall (+F)
         }
         if (i == ni - 1) {
-            if (a.bc[1] == BC_WALL) plus_i(li + 1, lj);
-            else minus_i(li + 1, lj);   // periodic wrap (-F on cell ni-1), ghost face (-F)
+            if (a.bc[1] == BC_WALL) plus(kir);
+            else minus(kir);   // periodic wrap (-F on cell ni-1), ghost face (-F)
         }
         if (j == 0) {
-            plus_j(li, lj);
+            plus(kjl);
         }
         if (j == nj - 1) {
-            if (a.bc[3] == BC_WALL) plus_j(li, lj + 1);
-            else minus_j(li, lj + 1);
+            if (a.bc[3] == BC_WALL) plus(kjr);
+            else minus(kjr);
         }
         const long long c = (long long)i * nj + j;
         if (a.mode == MODE_RESIDUAL) {
