@@ -110,21 +110,25 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
         ctx.synchronize()
         ctx.set_state(cfg["q"])
         ctx.synchronize()
+    # HIP events on the context's stream around the timed (graph-launched) steps
+    ctx.set_profiling(2)
     t0 = time.perf_counter()
     ctx.step(steps)
     ctx.synchronize()
     t1 = time.perf_counter()
+    region = ctx.profile()
+    ctx.set_profiling(0)
     q = ctx.get_state()
     st = ctx.status()
     ctx.status()   # clear
-    # live kernel timing with HIP events on the context's stream (no graphs while profiling)
+    # per-launch breakdown (events around every launch, graphs off)
     ctx.set_state(cfg["q"])
-    ctx.set_profiling(True)
+    ctx.set_profiling(1)
     ctx.step(min(steps, 50))
-    prof = ctx.profile()
-    ctx.set_profiling(False)
+    per_launch = ctx.profile()
+    ctx.set_profiling(0)
     ctx.close()
-    return t1 - t0, q, st, prof, warmup
+    return t1 - t0, q, st, (region, per_launch), warmup
 
 
 def cpu_baseline(cfg, budget_s=12.0):
@@ -172,6 +176,8 @@ def main():
     ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the divergence search and the face-frame Roe runs (for counter collection)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     a = ap.parse_args()
     if a.gpus != 1:
@@ -194,10 +200,13 @@ def main():
 
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
-        el, q, st, prof, warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
-        kern_ms = prof[0] / max(prof[1], 1.0)
-        res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms, upd_ms=prof[2] / max(prof[3], 1.0),
-                         rate=N * ns * steps / el)
+        el, q, st, (region, per_launch), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
+        # stage-kernel time: the events' interval over the timed steps per stage launch (it also
+        # holds the global-dt update kernels and the gaps between launches: an upper bound)
+        kern_ms = region[0] / max(region[1], 1.0)
+        res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms,
+                         launch_ms=per_launch[0] / max(per_launch[1], 1.0),
+                         upd_ms=per_launch[2] / max(per_launch[3], 1.0), rate=N * ns * steps / el)
         print(f"# {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, "
               f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}",
               file=sys.stderr, flush=True)
@@ -205,7 +214,7 @@ def main():
     # the reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:84-91) and the cylinder case
     # diverges after a few hundred steps in the reference algorithm itself: the mixed-vs-FP64 error is
     # measured over the window before the FP64 state leaves the finite numbers
-    diverge = first_nonfinite_step(cfg, cfg["nsteps"]) if a.workload == "c2" else None
+    diverge = first_nonfinite_step(cfg, cfg["nsteps"]) if a.workload == "c2" and not a.no_extras else None
     err_steps = steps if diverge is None or diverge - 50 > steps else max(50, diverge - 100)
     mixed_err = None
     if 0 in res and 1 in res:
@@ -219,7 +228,7 @@ def main():
 
     # the same workload with the face-frame Roe variant (stable on the O-grid)
     ff = None
-    if a.workload == "c2" and a.precision == "both":
+    if a.workload == "c2" and a.precision == "both" and not a.no_extras:
         import hyperboliconvergence_amd as hca
         cfg_ff = dict(cfg)
         cfg_ff["case"] = type(cs).from_buffer_copy(cs)
@@ -227,8 +236,8 @@ def main():
         ff = {}
         ff_steps = cfg["nsteps"]
         for prec in (0, 1):
-            el, q, st, prof, _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
-            ff[prec] = dict(rate=N * ns * ff_steps / el, q=q, status=st, kernel_us=prof[0] / max(prof[1], 1.0) * 1e3)
+            el, q, st, (region, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
+            ff[prec] = dict(rate=N * ns * ff_steps / el, q=q, status=st, kernel_us=region[0] / max(region[1], 1.0) * 1e3)
         ff_err = rel_err_conserved(ff[1]["q"], ff[0]["q"], cs.law, cs.basis)
         print(f"# face-frame Roe: fp64 {ff[0]['rate']:.4e}, mixed {ff[1]['rate']:.4e} cell-stage/s, "
               f"mixed vs fp64 after {ff_steps} steps {ff_err:.3e}, status {ff[0]['status']}/{ff[1]['status']}",
@@ -245,7 +254,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload, head),
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bytes_per_cell_stage,
-                "kernel_us": h["kernel_ms"] * 1e3}
+                "kernel_us": h["kernel_ms"] * 1e3, "kernel_us_eager_events": h["launch_ms"] * 1e3,
+                "update_kernel_us": h["upd_ms"] * 1e3 if cs.dt_mode == 0 else None}
 
     cpu = None
     if not a.no_cpu_baseline:

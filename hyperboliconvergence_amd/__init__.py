@@ -187,8 +187,10 @@ class Context:
     def set_graph_steps(self, k):
         self._check(self._L.hc_set_graph_steps(self._h, int(k)), "hc_set_graph_steps")
 
-    def set_profiling(self, on):
-        self._check(self._L.hc_set_profiling(self._h, 1 if on else 0), "hc_set_profiling")
+    def set_profiling(self, mode):
+        """0 off; 1 (or True) events around every launch, graphs off; 2 events around each step() call."""
+        m = int(mode) if not isinstance(mode, bool) else (1 if mode else 0)
+        self._check(self._L.hc_set_profiling(self._h, m), "hc_set_profiling")
 
     def profile(self):
         out = np.zeros(4)

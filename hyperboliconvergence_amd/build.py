@@ -33,9 +33,12 @@ def _jobs(mixed_contract="fast", build_dir=None):
     bdir = build_dir or BUILD
     for (law, basis, flux) in KERNELS:
         for prec in (64, 32):
-            contract = "-ffp-contract=off" if prec == 64 else f"-ffp-contract={mixed_contract}"
+            # FP64 objects: no contraction (bit-faithful); mixed objects: contraction and the
+            # hardware FP32 reciprocal / square root instead of the correctly rounded sequences
+            fpflags = (["-ffp-contract=off"] if prec == 64 else
+                       [f"-ffp-contract={mixed_contract}", "-fno-hip-fp32-correctly-rounded-divide-sqrt"])
             out = os.path.join(bdir, f"stage_L{law}_B{basis}_F{flux}_P{prec}.o")
-            cmd = [HIPCC, *COMMON, contract, f"-DHC_LAW={law}", f"-DHC_BASIS={basis}", f"-DHC_FLUX={flux}",
+            cmd = [HIPCC, *COMMON, *fpflags, f"-DHC_LAW={law}", f"-DHC_BASIS={basis}", f"-DHC_FLUX={flux}",
                    f"-DHC_PREC={prec}", "-x", "hip", "-c", os.path.join(CSRC, "hc_kernel_inst.hip"), "-o", out]
             objs.append((out, cmd))
     for src in ("hc_geom.cpp", "hc_capi.cpp"):

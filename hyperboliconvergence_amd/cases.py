@@ -231,3 +231,75 @@ def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):
     fname = "slau" if flux == FLUX_SLAU else "ausmpup"
     return dict(name=f"euler-isentropic-vortex-{n}x{n}-{fname}-vanalbada-ssprk3", nodes=nodes, q=q, case=cs,
                 bref=np.zeros((bref_size(n, n), 4)), nsteps=nsteps)
+
+
+# ---------------- batch of cylinder cases ----------------
+
+class MT19937_64:
+    """64-bit Mersenne Twister (Matsumoto & Nishimura 2004), the engine of std::mt19937_64:
+    w=64, n=312, m=156, r=31, tempering (u,d,s,b,t,c,l) = (29, 0x5555555555555555, 17,
+    0x71D67FFFEDA60000, 37, 0xFFF7EEE000000000, 43), seeding multiplier 6364136223846793005."""
+    N, M = 312, 156
+    MATRIX_A = 0xB5026F5AA96619E9
+    UPPER, LOWER = 0xFFFFFFFF80000000, 0x7FFFFFFF
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed=5489):
+        mt = [0] * self.N
+        mt[0] = seed & self.MASK
+        for i in range(1, self.N):
+            mt[i] = (6364136223846793005 * (mt[i - 1] ^ (mt[i - 1] >> 62)) + i) & self.MASK
+        self.mt, self.idx = mt, self.N
+
+    def _twist(self):
+        mt, N, M = self.mt, self.N, self.M
+        for i in range(N):
+            x = (mt[i] & self.UPPER) | (mt[(i + 1) % N] & self.LOWER)
+            xa = x >> 1
+            if x & 1:
+                xa ^= self.MATRIX_A
+            mt[i] = mt[(i + M) % N] ^ xa
+        self.idx = 0
+
+    def __call__(self):
+        if self.idx >= self.N:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & self.MASK
+
+
+BATCH_SIZES = ((128, 64), (256, 128), (384, 192), (512, 256))   # (n_theta, n_r)
+
+
+def batch_case_table(ncases=96, seed=20261202):
+    """Cylinder batch: per case, in draw order, size index = rng() % 4, Mach = 0.2 + 0.6 U and
+    angle of attack = -5 + 10 U degrees, U = (rng() >> 11) * 2^-53."""
+    rng = MT19937_64(seed)
+    rows = []
+    for k in range(ncases):
+        size = rng() % 4
+        mach = 0.2 + 0.6 * ((rng() >> 11) * 2.0 ** -53)
+        alpha = -5.0 + 10.0 * ((rng() >> 11) * 2.0 ** -53)
+        nth, nr = BATCH_SIZES[size]
+        rows.append(dict(case=k, size=size, nth=nth, nr=nr, mach=mach, alpha_deg=alpha))
+    return rows
+
+
+def config_cylinder(nth, nr, mach, alpha_deg=0.0, flux=FLUX_ROE, nsteps=1000):
+    """One case of the cylinder batch: the O-grid of config_c2 at (n_theta, n_r), freestream
+    (u, v) = M (cos a, sin a), rho = 1, p = 1/gamma; Roe, MinMod2, LSQ, RK4, local dt at CFL 1."""
+    nodes = mesh_ogrid(nr, nth)
+    a = math.radians(alpha_deg)
+    qinf = np.array([mach * math.cos(a), mach * math.sin(a), 1.0, 1.0 / GAMMA])
+    q = np.broadcast_to(qinf, (nr, nth, 4)).copy()
+    bref = np.broadcast_to(qinf, (bref_size(nr, nth), 4)).copy()
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, flux, LIM_MINMOD2, (BC_WALL, BC_RIEMANN, BC_PERIODIC, BC_PERIODIC),
+                   "rk4", 1.0, DT_LOCAL, air_species(mach))
+    return dict(name=f"euler-cylinder-{nth}x{nr}-M{mach:.3f}-a{alpha_deg:+.2f}", nodes=nodes, q=q, case=cs,
+                bref=bref, nsteps=nsteps)
+

@@ -141,7 +141,12 @@ struct hc_context {
     int profiling = 0;
     double prof[4] = {0, 0, 0, 0};
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    struct Pending {
+        hipEvent_t e0, e1;
+        double n_stage, n_update;   // launches the interval covers
+        int kind;                   // 0 stage kernel, 1 update kernel, 2 whole hc_step call
+    };
+    std::vector<Pending> ev_pending;
 };
 
 namespace {
@@ -289,15 +294,15 @@ hipEvent_t take_event(hc_context* ctx)
     return e;
 }
 
-// kind: 0 stage kernel, 1 update kernel
+// profiling mode 1: events around every launch; kind 0 stage kernel, 1 update kernel
 template <class F> hipError_t timed(hc_context* ctx, int kind, F&& launch)
 {
-    if (!ctx->profiling) return launch();
+    if (ctx->profiling != 1) return launch();
     hipEvent_t e0 = take_event(ctx), e1 = take_event(ctx);
     (void)hipEventRecord(e0, ctx->stream);
     const hipError_t e = launch();
     (void)hipEventRecord(e1, ctx->stream);
-    ctx->ev_pending.push_back({kind, {e0, e1}});
+    ctx->ev_pending.push_back({e0, e1, kind == 0 ? 1.0 : 0.0, kind == 1 ? 1.0 : 0.0, kind});
     return e;
 }
 
@@ -305,12 +310,13 @@ void drain_events(hc_context* ctx)
 {
     for (auto& p : ctx->ev_pending) {
         float ms = 0.f;
-        (void)hipEventSynchronize(p.second.second);
-        (void)hipEventElapsedTime(&ms, p.second.first, p.second.second);
-        ctx->prof[2 * p.first] += ms;
-        ctx->prof[2 * p.first + 1] += 1.0;
-        ctx->ev_pool.push_back(p.second.first);
-        ctx->ev_pool.push_back(p.second.second);
+        (void)hipEventSynchronize(p.e1);
+        (void)hipEventElapsedTime(&ms, p.e0, p.e1);
+        ctx->prof[p.kind == 1 ? 2 : 0] += ms;
+        ctx->prof[1] += p.n_stage;
+        ctx->prof[3] += p.n_update;
+        ctx->ev_pool.push_back(p.e0);
+        ctx->ev_pool.push_back(p.e1);
     }
     ctx->ev_pending.clear();
 }
@@ -432,8 +438,8 @@ int hc_destroy(hc_context* ctx)
     free_dev(ctx->part);
     free_dev(ctx->nout);
     for (auto& p : ctx->ev_pending) {
-        (void)hipEventDestroy(p.second.first);
-        (void)hipEventDestroy(p.second.second);
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
     }
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -641,7 +647,12 @@ int hc_step(hc_context* ctx, int nsteps)
     int k = ctx->graph_steps;
     // single-stage tables alternate two state buffers: capture an even number of steps
     if (ctx->cs.nstages == 1 && (k % 2)) k += 1;
-    if (k > 0 && !ctx->profiling) {
+    hipEvent_t r0 = nullptr;
+    if (ctx->profiling == 2 && nsteps > 0) {
+        r0 = take_event(ctx);
+        (void)hipEventRecord(r0, ctx->stream);
+    }
+    if (k > 0 && ctx->profiling != 1) {
         if (ctx->gexec && ctx->graph_cur != ctx->cur) {
             (void)hipGraphExecDestroy(ctx->gexec);
             (void)hipGraphDestroy(ctx->graph);
@@ -671,7 +682,13 @@ int hc_step(hc_context* ctx, int nsteps)
         r = enqueue_step(ctx);
         if (r) return r;
     }
-    if (ctx->profiling) drain_events(ctx);
+    if (r0) {
+        hipEvent_t r1 = take_event(ctx);
+        (void)hipEventRecord(r1, ctx->stream);
+        const double upd = ctx->cs.dt_mode == 0 ? 1.0 : 0.0;
+        ctx->ev_pending.push_back({r0, r1, double(nsteps) * ctx->cs.nstages, double(nsteps) * upd, 2});
+    }
+    if (ctx->profiling == 1) drain_events(ctx);
     return HC_OK;
 }
 
@@ -719,7 +736,8 @@ int hc_set_graph_steps(hc_context* ctx, int k)
 int hc_set_profiling(hc_context* ctx, int enable)
 {
     if (!ctx) return HC_ERR_ARG;
-    ctx->profiling = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return fail(ctx, HC_ERR_ARG, "profiling mode must be 0, 1 or 2");
+    ctx->profiling = enable;
     for (double& p : ctx->prof) p = 0.0;
     return HC_OK;
 }

@@ -17,10 +17,10 @@
 
 #if HC_PREC == 64
 using RF = double;
-constexpr int TI = hc::TI64, TJ = hc::TJ64;
+constexpr int TI = hc::TI64, TJ = hc::TJ64, NT = hc::NT64;
 #else
 using RF = float;
-constexpr int TI = hc::TI32, TJ = hc::TJ32;
+constexpr int TI = hc::TI32, TJ = hc::TJ32, NT = hc::NT32;
 #endif
 
 hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::StageArgs<RF>& a, hipStream_t st)
@@ -28,7 +28,7 @@ hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::StageArgs<RF>& 
     const int tiles_i = (a.m.ni + TI - 1) / TI;
     const int tiles_j = (a.m.nj + TJ - 1) / TJ;
     const int ntiles = tiles_i * tiles_j;
-    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, TI, TJ>), dim3(ntiles), dim3(TI * TJ), 0, st, a,
+    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, TI, TJ, NT>), dim3(ntiles), dim3(NT), 0, st, a,
                        tiles_j, ntiles);
     return hipGetLastError();
 }

@@ -3,12 +3,10 @@
 #pragma once
 
 #include "hc_stage.hpp"
+#include "hc_tiles.hpp"
 
 namespace hc {
 
-// tile shapes: FP64 face arithmetic 4x32 (128 lanes), mixed 8x32 (256 lanes)
-constexpr int TI64 = 4, TJ64 = 32;
-constexpr int TI32 = 8, TJ32 = 32;
 
 using StageLaunch64 = hipError_t (*)(const StageArgs<double>&, hipStream_t);
 using StageLaunch32 = hipError_t (*)(const StageArgs<float>&, hipStream_t);

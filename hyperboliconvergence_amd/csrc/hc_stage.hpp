@@ -153,9 +153,13 @@ HC_HD int xcd_tile(int b, int ntiles)
     return start + y;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTile {
+// NTHR threads per work-group, at least one per cell: the extra lanes share the load, metric and
+// face phases, so more waves are resident per SIMD for the same LDS tile
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ> struct StageTile {
     static constexpr int NV = LawTraits<LAW>::nv;
-    static constexpr int NT = TI * TJ;
+    static constexpr int NC = TI * TJ;   // cells of the tile
+    static constexpr int NT = NTHR;      // threads of the work-group
+    static_assert(NT >= NC, "one thread per cell at least");
     static constexpr int QH = TI + 4, QW = TJ + 4;   // q tile with 2-cell halo
     static constexpr int MH = TI + 2, MW = TJ + 2;   // Q ring with 1-cell halo
     static constexpr int NFI = (TI + 1) * TJ;
@@ -368,7 +372,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> struct StageTi
         const int li = tid / TJ, lj = tid % TJ;
         const int i = i0 + li, j = j0 + lj;
         s.red[tid] = 0.0;
-        if (i >= ni || j >= nj) {
+        if (tid >= NC || i >= ni || j >= nj) {
             s.flg[tid] = flag;
             return;
         }
@@ -508,10 +512,10 @@ __device__ inline unsigned int wave_or(unsigned int v)
     return v;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ>
-__global__ void __launch_bounds__(TI * TJ) k_stage(StageArgs<RF> a, int tiles_j, int ntiles)
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR>
+__global__ void __launch_bounds__(NTHR) k_stage(StageArgs<RF> a, int tiles_j, int ntiles)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the

@@ -148,9 +148,11 @@ int hc_synchronize(hc_context* ctx);
 int hc_set_graph_steps(hc_context* ctx, int k);
 
 /*
- * Kernel timing: when enabled, hc_step records HIP events around each launch;
- * hc_get_profile returns {stage kernel ms, stage-kernel launches, update kernel ms,
- * update launches}.  Used for the roofline report.
+ * Kernel timing on the context's stream with HIP events.  mode 1: events around every
+ * launch (graphs off); mode 2: one event pair around each hc_step call (graphs on), the
+ * interval counted against the stage kernels it launched.  hc_get_profile returns the sums
+ * {stage kernel ms, stage-kernel launches, update kernel ms, update launches} since the last
+ * hc_set_profiling.  Used for the roofline report.
  */
 int hc_set_profiling(hc_context* ctx, int enable);
 int hc_get_profile(hc_context* ctx, double* out4);

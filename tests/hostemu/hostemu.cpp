@@ -7,6 +7,7 @@
 // never part of the product library.
 #include "hc_geom.hpp"
 #include "hc_stage.hpp"
+#include "hc_tiles.hpp"
 #include "hcamd.h"
 
 #include <cfloat>
@@ -72,9 +73,9 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
     return a;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ> void run_stage(const StageArgs<RF>& a)
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT> void run_stage(const StageArgs<RF>& a)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT>;
     const int tiles_i = (a.m.ni + TI - 1) / TI, tiles_j = (a.m.nj + TJ - 1) / TJ;
     const int ntiles = tiles_i * tiles_j;
     auto sp = std::make_unique<typename T::Smem>();
@@ -107,24 +108,24 @@ template <int LAW, int BASIS, class RF> void run_update(const StageArgs<RF>& a)
     *a.status |= flag;
 }
 
-template <class RF, int TI, int TJ> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
+template <class RF, int TI, int TJ, int NT> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
 {
     const int key = cs.law * 100 + cs.basis * 10 + cs.flux;
     switch (key) {
-    case 0: run_stage<0, 0, 0, RF, TI, TJ>(a); break;
-    case 1: run_stage<0, 0, 1, RF, TI, TJ>(a); break;
-    case 100: run_stage<1, 0, 0, RF, TI, TJ>(a); break;
-    case 101: run_stage<1, 0, 1, RF, TI, TJ>(a); break;
-    case 102: run_stage<1, 0, 2, RF, TI, TJ>(a); break;
-    case 103: run_stage<1, 0, 3, RF, TI, TJ>(a); break;
-    case 104: run_stage<1, 0, 4, RF, TI, TJ>(a); break;
-    case 105: run_stage<1, 0, 5, RF, TI, TJ>(a); break;
-    case 110: run_stage<1, 1, 0, RF, TI, TJ>(a); break;
-    case 111: run_stage<1, 1, 1, RF, TI, TJ>(a); break;
-    case 112: run_stage<1, 1, 2, RF, TI, TJ>(a); break;
-    case 113: run_stage<1, 1, 3, RF, TI, TJ>(a); break;
-    case 114: run_stage<1, 1, 4, RF, TI, TJ>(a); break;
-    case 115: run_stage<1, 1, 5, RF, TI, TJ>(a); break;
+    case 0: run_stage<0, 0, 0, RF, TI, TJ, NT>(a); break;
+    case 1: run_stage<0, 0, 1, RF, TI, TJ, NT>(a); break;
+    case 100: run_stage<1, 0, 0, RF, TI, TJ, NT>(a); break;
+    case 101: run_stage<1, 0, 1, RF, TI, TJ, NT>(a); break;
+    case 102: run_stage<1, 0, 2, RF, TI, TJ, NT>(a); break;
+    case 103: run_stage<1, 0, 3, RF, TI, TJ, NT>(a); break;
+    case 104: run_stage<1, 0, 4, RF, TI, TJ, NT>(a); break;
+    case 105: run_stage<1, 0, 5, RF, TI, TJ, NT>(a); break;
+    case 110: run_stage<1, 1, 0, RF, TI, TJ, NT>(a); break;
+    case 111: run_stage<1, 1, 1, RF, TI, TJ, NT>(a); break;
+    case 112: run_stage<1, 1, 2, RF, TI, TJ, NT>(a); break;
+    case 113: run_stage<1, 1, 3, RF, TI, TJ, NT>(a); break;
+    case 114: run_stage<1, 1, 4, RF, TI, TJ, NT>(a); break;
+    case 115: run_stage<1, 1, 5, RF, TI, TJ, NT>(a); break;
     default: break;
     }
 }
@@ -140,8 +141,8 @@ template <class RF> void dispatch_update(const hc_case_desc& cs, const StageArgs
 // tile shapes of hc_launch.hpp: FP64 4x32, mixed 8x32
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
-    if (e.precision == HC_PRECISION_MIXED) dispatch_stage<float, 8, 32>(e.cs, a32);
-    else dispatch_stage<double, 4, 32>(e.cs, a64);
+    if (e.precision == HC_PRECISION_MIXED) dispatch_stage<float, TI32, TJ32, NT32>(e.cs, a32);
+    else dispatch_stage<double, TI64, TJ64, NT64>(e.cs, a64);
 }
 
 void update(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
