@@ -118,6 +118,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     t1 = time.perf_counter()
     region = ctx.profile()
     ctx.set_profiling(0)
+    tiles = ctx.tiles()
     q = ctx.get_state()
     st = ctx.status()
     ctx.status()   # clear
@@ -128,7 +129,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     per_launch = ctx.profile()
     ctx.set_profiling(0)
     ctx.close()
-    return t1 - t0, q, st, (region, per_launch), warmup
+    return t1 - t0, q, st, (region, per_launch, tiles), warmup
 
 
 def cpu_baseline(cfg, budget_s=12.0):
@@ -168,12 +169,146 @@ def traffic_from_profiles(workload, precision):
         return None
 
 
+def batch_cfgs(flux=None):
+    """The 96-case cylinder batch (table drawn with MT19937-64 seed 20261202)."""
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    flux = hca.FLUX_ROE if flux is None else flux
+    return [cases.config_cylinder(r["nth"], r["nr"], r["mach"], r["alpha_deg"], flux=flux)
+            for r in cases.batch_case_table()]
+
+
+def run_batch(cfgs, precision, steps, warmup, graph_steps):
+    import hyperboliconvergence_amd as hca
+    b = hca.Batch(0)
+    b.begin()
+    for cfg in cfgs:
+        b.add_case(cfg["nodes"], cfg["case"])
+    b.end()
+    b.set_precision(precision)
+    b.set_graph_steps(graph_steps)
+
+    def reset():
+        for k, cfg in enumerate(cfgs):
+            b.set_case_boundary_refs(k, cfg["bref"])
+            b.set_case_state(k, cfg["q"])
+        b.synchronize()
+
+    reset()
+    if graph_steps > 0 and warmup % graph_steps:
+        warmup += graph_steps - warmup % graph_steps
+    if warmup:
+        b.step(warmup)
+        b.synchronize()
+        reset()
+    b.set_profiling(2)
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.synchronize()
+    t1 = time.perf_counter()
+    region = b.profile()
+    b.set_profiling(0)
+    tiles = b.tiles()
+    qs = [b.get_case_state(k) for k in range(len(cfgs))]
+    st = b.status()
+    b.close()
+    return t1 - t0, qs, st, region, tiles, warmup
+
+
+def cpu_baseline_batch(cfgs, budget_s=12.0):
+    """The oracle's fast build on the first case of each grid size of the batch, bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    po.set_threads(threads, fast=True)
+    seen, picks = set(), []
+    for k, cfg in enumerate(cfgs):
+        shape = cfg["q"].shape[:2]
+        if shape not in seen:
+            seen.add(shape)
+            picks.append(k)
+    work, secs, desc = 0.0, 0.0, []
+    for k in picks:
+        cfg = cfgs[k]
+        ni, nj = cfg["q"].shape[:2]
+        _, _, tm1 = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 1, cfg["bref"], fast=True)
+        per_step = max(tm1.sum() / 1e3, 1e-6)
+        n = int(max(1, min(cfg["nsteps"], budget_s / len(picks) / per_step)))
+        _, _, tm = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], n, cfg["bref"], fast=True)
+        work += ni * nj * cfg["case"].nstages * n
+        secs += tm.sum() / 1e3
+        desc.append(f"case {k} ({nj}x{ni}) {n} steps")
+    return {"value": work / secs, "unit": "cell-stage updates/s", "cores": threads, "kind": "port",
+            "sample": "cylinder batch, first case of each grid size: " + ", ".join(desc) +
+                      f"; oracle built -O3 -ffast-math -fopenmp (cpp/Makefile:38 flags), {threads} OpenMP threads"}
+
+
+def main_batch(a):
+    import hyperboliconvergence_amd as hca
+    cfgs = batch_cfgs()
+    steps = a.steps if a.steps is not None else 300
+    Ntot = sum(c["q"].shape[0] * c["q"].shape[1] for c in cfgs)
+    ns = cfgs[0]["case"].nstages
+    res = {}
+    for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
+        el, qs, st, region, tiles, warmup = run_batch(cfgs, prec, steps, a.warmup, a.graph_steps)
+        res[prec] = dict(seconds=el, qs=qs, status=st, kernel_ms=region[0] / max(region[1], 1.0),
+                         rate=Ntot * ns * steps / el, tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]])
+        print(f"# batch {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, {res[prec]['rate']:.4e} "
+              f"cell-stage/s, stage kernel {res[prec]['kernel_ms'] * 1e3:.1f} us, status {st}, "
+              f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})", file=sys.stderr, flush=True)
+    mixed_err = None
+    if 0 in res and 1 in res:
+        errs = [rel_err_conserved(q1, q0, 1, 1) for q0, q1 in zip(res[0]["qs"], res[1]["qs"])]
+        mixed_err = float(np.max(errs)) if np.all(np.isfinite(errs)) else float("nan")
+        print(f"# batch mixed vs fp64 after {steps} steps: max over cases {mixed_err:.3e}", file=sys.stderr, flush=True)
+    head = 1 if (1 in res and mixed_err is not None and mixed_err <= 1e-6) else 0
+    h = res[head]
+    ff = None
+    if a.precision == "both" and not a.no_extras:
+        cfgs_ff = batch_cfgs(hca.FLUX_ROE_FACEFRAME)
+        ff_steps = 1000
+        ff = {"flux": "RoeFluxFaceFrame", "steps": ff_steps}
+        for prec in (0, 1):
+            el, qs, st, region, _, _ = run_batch(cfgs_ff, prec, ff_steps, a.warmup, a.graph_steps)
+            ff["mixed_value" if prec else "fp64_value"] = Ntot * ns * ff_steps / el
+            ff["mixed_status" if prec else "fp64_status"] = int(st)
+            ff["q%d" % prec] = qs
+        errs = [rel_err_conserved(q1, q0, 1, 1) for q0, q1 in zip(ff.pop("q0"), ff.pop("q1"))]
+        ff["mixed_vs_fp64_rel_err"] = float(np.max(errs))
+        print(f"# batch face-frame Roe: fp64 {ff['fp64_value']:.4e}, mixed {ff['mixed_value']:.4e} cell-stage/s "
+              f"over {ff_steps} steps, mixed vs fp64 {ff['mixed_vs_fp64_rel_err']:.3e}", file=sys.stderr, flush=True)
+    with hca.Context(0) as c:
+        triad = c.stream_triad(1 << 26, 20)
+    bpc = 112.0
+    achieved = bpc * Ntot / (h["kernel_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("c4", head),
+                "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bpc, "kernel_us": h["kernel_ms"] * 1e3}
+    cpu = None if a.no_cpu_baseline else cpu_baseline_batch(cfgs)
+    out = {
+        "metric": "cell-stage updates/s + %HBM roofline; mixed-FP32 vs FP64 at ≤1e-6 rel err",
+        "value": h["rate"], "unit": "cell-stage updates/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": h["seconds"] / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-face/f64-state" if head == 1 else "f64", "data": "synthetic",
+        "config": {"workload": "euler-cylinder-batch-96-roe-minmod-lsq-rk4-local", "cases": len(cfgs),
+                   "cells": Ntot, "rk_stages": ns, "precision": "mixed" if head == 1 else "fp64",
+                   "graph_steps": a.graph_steps, "configured_steps": 1000, "tile_shape": h["tiles"],
+                   "tile_shape_ms_per_step": h["tile_ms"]},
+        "roofline": roofline, "cpu_baseline": cpu,
+        "fp64_value": res[0]["rate"] if 0 in res else None, "mixed_value": res[1]["rate"] if 1 in res else None,
+        "mixed_vs_fp64_rel_err": mixed_err, "status": int(h["status"]), "face_frame_roe": ff,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -188,6 +323,8 @@ def main():
         print("bench.py runs on one GPU only", file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "c4":
+        return main_batch(a)
     cfg = make_cfg(a.workload, a.steps)
     steps = a.steps if a.steps is not None else cfg["bench_steps"]
     cs = cfg["case"]
@@ -200,15 +337,17 @@ def main():
 
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
-        el, q, st, (region, per_launch), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
+        el, q, st, (region, per_launch, tiles), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
         # stage-kernel time: the events' interval over the timed steps per stage launch (it also
         # holds the global-dt update kernels and the gaps between launches: an upper bound)
         kern_ms = region[0] / max(region[1], 1.0)
         res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms,
                          launch_ms=per_launch[0] / max(per_launch[1], 1.0),
-                         upd_ms=per_launch[2] / max(per_launch[3], 1.0), rate=N * ns * steps / el)
+                         upd_ms=per_launch[2] / max(per_launch[3], 1.0), rate=N * ns * steps / el,
+                         tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]])
         print(f"# {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, "
-              f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}",
+              f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}, "
+              f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})",
               file=sys.stderr, flush=True)
 
     # the reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:84-91) and the cylinder case
@@ -236,7 +375,7 @@ def main():
         ff = {}
         ff_steps = cfg["nsteps"]
         for prec in (0, 1):
-            el, q, st, (region, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
+            el, q, st, (region, _, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
             ff[prec] = dict(rate=N * ns * ff_steps / el, q=q, status=st, kernel_us=region[0] / max(region[1], 1.0) * 1e3)
         ff_err = rel_err_conserved(ff[1]["q"], ff[0]["q"], cs.law, cs.basis)
         print(f"# face-frame Roe: fp64 {ff[0]['rate']:.4e}, mixed {ff[1]['rate']:.4e} cell-stage/s, "
@@ -278,7 +417,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": cfg["name"], "cells": N, "ni": ni, "nj": nj, "rk_stages": ns,
                    "precision": "mixed" if head == 1 else "fp64", "graph_steps": a.graph_steps,
-                   "configured_steps": cfg["nsteps"]},
+                   "configured_steps": cfg["nsteps"], "tile_shape": h["tiles"],
+                   "tile_shape_ms_per_step": h["tile_ms"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_value": res[0]["rate"] if 0 in res else None,

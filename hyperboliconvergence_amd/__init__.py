@@ -29,7 +29,11 @@ C_ABI_SYMBOLS = (
     "hc_set_boundary_refs", "hc_set_state", "hc_get_state", "hc_residual", "hc_step", "hc_get_time",
     "hc_norms", "hc_get_status", "hc_synchronize", "hc_set_graph_steps", "hc_set_profiling",
     "hc_get_profile", "hc_device_state", "hc_stream_triad", "hc_version",
+    "hc_batch_begin", "hc_batch_add_case", "hc_batch_end", "hc_batch_size", "hc_batch_set_boundary_refs",
+    "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
+    "hc_set_tiles", "hc_get_tiles",
 )
+NTILECFG = 2
 
 
 class Species(C.Structure):
@@ -84,6 +88,18 @@ def library():
         "hc_device_state": (i, [P, P]),
         "hc_stream_triad": (i, [P, C.c_longlong, i, P]),
         "hc_version": (C.c_char_p, []),
+        "hc_batch_begin": (i, [P]),
+        "hc_batch_add_case": (i, [P, i, i, P, P, P]),
+        "hc_batch_end": (i, [P]),
+        "hc_batch_size": (i, [P, P]),
+        "hc_batch_set_boundary_refs": (i, [P, i, P]),
+        "hc_batch_set_state": (i, [P, i, P]),
+        "hc_batch_get_state": (i, [P, i, P]),
+        "hc_batch_residual": (i, [P, i, P]),
+        "hc_batch_norms": (i, [P, i, P, P]),
+        "hc_batch_get_time": (i, [P, i, P]),
+        "hc_set_tiles": (i, [P, i]),
+        "hc_get_tiles": (i, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -197,6 +213,17 @@ class Context:
         self._check(self._L.hc_get_profile(self._h, _p(out)), "hc_get_profile")
         return out
 
+    def set_tiles(self, config):
+        """-1: time every tile shape on the first steps and keep the fastest; 0, 1: fixed shape."""
+        self._check(self._L.hc_set_tiles(self._h, int(config)), "hc_set_tiles")
+
+    def tiles(self):
+        """(shape in use, milliseconds per step of each shape once timed)."""
+        cfg = C.c_int()
+        ms = np.zeros(NTILECFG)
+        self._check(self._L.hc_get_tiles(self._h, C.byref(cfg), _p(ms)), "hc_get_tiles")
+        return cfg.value, ms
+
     def stream_triad(self, n, reps):
         g = C.c_double()
         self._check(self._L.hc_stream_triad(self._h, int(n), int(reps), C.byref(g)), "hc_stream_triad")
@@ -210,3 +237,67 @@ class Context:
 
 def version():
     return library().hc_version().decode()
+
+
+class Batch(Context):
+    """A context holding several cases that share flux, limiter, basis and Runge-Kutta table;
+    step() advances all of them with one launch per stage (hc_batch_* of include/hcamd.h)."""
+
+    def __init__(self, device=0):
+        super().__init__(device)
+        self.shapes = []
+
+    def begin(self):
+        self._check(self._L.hc_batch_begin(self._h), "hc_batch_begin")
+        self.shapes = []
+
+    def add_case(self, nodes, desc):
+        nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+        ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+        cid = C.c_int()
+        self._check(self._L.hc_batch_add_case(self._h, ni, nj, _p(nodes), C.cast(C.pointer(desc), C.c_void_p),
+                                              C.byref(cid)), "hc_batch_add_case")
+        self.shapes.append((ni, nj))
+        self.nv = 4 if desc.law == LAW_EULER else 3
+        return cid.value
+
+    def end(self):
+        self._check(self._L.hc_batch_end(self._h), "hc_batch_end")
+
+    def __len__(self):
+        return len(self.shapes)
+
+    def set_case_boundary_refs(self, k, bref):
+        ni, nj = self.shapes[k]
+        bref = np.ascontiguousarray(bref, dtype=np.float64)
+        assert bref.size == (2 * nj + 2 * ni) * self.nv
+        self._check(self._L.hc_batch_set_boundary_refs(self._h, int(k), _p(bref)), "hc_batch_set_boundary_refs")
+
+    def set_case_state(self, k, q):
+        ni, nj = self.shapes[k]
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == ni * nj * self.nv
+        self._check(self._L.hc_batch_set_state(self._h, int(k), _p(q)), "hc_batch_set_state")
+
+    def get_case_state(self, k):
+        ni, nj = self.shapes[k]
+        q = np.zeros((ni, nj, self.nv))
+        self._check(self._L.hc_batch_get_state(self._h, int(k), _p(q)), "hc_batch_get_state")
+        return q
+
+    def case_residual(self, k):
+        ni, nj = self.shapes[k]
+        r = np.zeros((ni, nj, self.nv + 1))
+        self._check(self._L.hc_batch_residual(self._h, int(k), _p(r)), "hc_batch_residual")
+        return r
+
+    def case_norms(self, k):
+        l2, linf = np.zeros(self.nv), np.zeros(self.nv)
+        self._check(self._L.hc_batch_norms(self._h, int(k), _p(l2), _p(linf)), "hc_batch_norms")
+        return l2, linf
+
+    def case_time(self, k):
+        t = C.c_double()
+        self._check(self._L.hc_batch_get_time(self._h, int(k), C.byref(t)), "hc_batch_get_time")
+        return t.value
+

@@ -28,7 +28,7 @@ COMMON = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-I", CSRC, "-
 KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(6)]
 
 
-def _jobs(mixed_contract="fast", build_dir=None):
+def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0)):
     objs = []
     bdir = build_dir or BUILD
     for (law, basis, flux) in KERNELS:
@@ -37,6 +37,9 @@ def _jobs(mixed_contract="fast", build_dir=None):
             # hardware FP32 reciprocal / square root instead of the correctly rounded sequences
             fpflags = (["-ffp-contract=off"] if prec == 64 else
                        [f"-ffp-contract={mixed_contract}", "-fno-hip-fp32-correctly-rounded-divide-sqrt"])
+            wpe = waves_per_eu[0] if prec == 64 else waves_per_eu[1]
+            if wpe:
+                fpflags.append(f"-DHC_STAGE_WAVES_PER_EU={wpe}")
             out = os.path.join(bdir, f"stage_L{law}_B{basis}_F{flux}_P{prec}.o")
             cmd = [HIPCC, *COMMON, *fpflags, f"-DHC_LAW={law}", f"-DHC_BASIS={basis}", f"-DHC_FLUX={flux}",
                    f"-DHC_PREC={prec}", "-x", "hip", "-c", os.path.join(CSRC, "hc_kernel_inst.hip"), "-o", out]
@@ -56,11 +59,12 @@ def _run(job):
     return out
 
 
-def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=None):
-    """Compile every HIP source and link libhcamd.so; always recompiles."""
+def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0)):
+    """Compile every HIP source and link libhcamd.so; always recompiles.
+    waves_per_eu: occupancy hint (waves per SIMD) of the FP64 / mixed stage kernels, 0 = none."""
     bdir = build_dir or BUILD
     os.makedirs(bdir, exist_ok=True)
-    work = _jobs(mixed_contract, bdir)
+    work = _jobs(mixed_contract, bdir, waves_per_eu)
     n = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=n) as ex:
         objs = list(ex.map(_run, work))
@@ -80,7 +84,8 @@ if __name__ == "__main__":
     ap.add_argument("--jobs", type=int, default=None)
     ap.add_argument("--out", default=LIB)
     ap.add_argument("--mixed-contract", default="fast", choices=["fast", "on", "off"])
+    ap.add_argument("--waves-per-eu", type=int, nargs=2, default=(0, 0), metavar=("FP64", "MIXED"))
     a = ap.parse_args()
     build(a.jobs, verbose=True, lib=a.out, mixed_contract=a.mixed_contract,
-          build_dir=None if a.out == LIB else os.path.join(BUILD, "alt"))
+          build_dir=None if a.out == LIB else os.path.join(BUILD, "alt"), waves_per_eu=tuple(a.waves_per_eu))
     sys.exit(0)

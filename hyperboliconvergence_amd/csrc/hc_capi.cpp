@@ -1,5 +1,8 @@
-// hc_capi.cpp -- the C-ABI of include/hcamd.h: context, device buffers, the
-// per-step launch sequence of the stage kernels, HIP graph capture, norms.
+// hc_capi.cpp -- the C-ABI of include/hcamd.h: context, its cases and device buffers, the
+// per-step launch sequence of the stage kernels, tile-shape autotuning, HIP graph capture,
+// norms.  A context holds one case (hc_set_mesh / hc_set_case) or a batch of cases
+// (hc_batch_begin / hc_batch_add_case / hc_batch_end) that share the flux, limiter, basis
+// and Runge-Kutta table; every stage launch covers the tiles of all its cases.
 #include "hcamd.h"
 
 #include "hc_geom.hpp"
@@ -17,7 +20,17 @@ namespace {
 
 using namespace hc;
 
-__global__ void k_reset_max(unsigned long long* p) { *p = dbl_bits(DBL_MIN); }
+__global__ void k_reset_max(unsigned long long* p, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = dbl_bits(DBL_MIN);
+}
+
+__global__ void k_clear_dt(double* p, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0.0;
+}
 
 __global__ void k_clear_u32(unsigned int* p) { *p = 0u; }
 
@@ -109,19 +122,41 @@ template <class T> void free_dev(T*& p)
 
 }   // namespace
 
+
+namespace {
+
+// host-side record of one case of a context
+struct CaseHost {
+    int ni = 0, nj = 0;
+    hc_case_desc cs{};
+    GeomOutput g;                  // host geometry until it is uploaded
+    long long N = 0, nfi = 0, nfj = 0;
+    long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
+    int brows = 0;                 // boundary reference rows 2 nj + 2 ni
+    bool have_state = false;
+    int tiles_j[NTILECFG] = {}, tile0[NTILECFG] = {}, ntiles[NTILECFG] = {};
+};
+
+}   // namespace
+
 struct hc_context {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
-    // mesh and case
+    // single-case staging (hc_set_mesh)
     int ni = 0, nj = 0;
     std::vector<double> nodes;
-    bool have_mesh = false, have_case = false, have_state = false;
-    hc_case_desc cs{};
+    bool have_mesh = false;
+    // cases
+    bool building = false;         // between hc_batch_begin and hc_batch_end
+    std::vector<CaseHost> cases;
+    bool have_case = false;        // device layout built
+    hc_case_desc cs{};             // parameters shared by the cases (case 0's descriptor)
     int nv = 0;
     int precision = HC_PRECISION_FP64;
     std::vector<RkPlan> plans;
     int nslots = 0;
+    long long Ntot = 0, nfi_tot = 0, nfj_tot = 0, brows_tot = 0;
     // device buffers
     double *cvol = nullptr, *ccx = nullptr, *ccy = nullptr, *fi = nullptr, *fj = nullptr;
     float *fi32 = nullptr, *fj32 = nullptr;
@@ -129,14 +164,25 @@ struct hc_context {
     int cur = 0;
     double *acc = nullptr, *tmp = nullptr, *lam0 = nullptr, *bref = nullptr, *res = nullptr, *part = nullptr,
            *nout = nullptr;
-    unsigned long long* maxslot = nullptr;
-    double* dtslot = nullptr;
+    unsigned long long* maxslot = nullptr;   // one per case
+    double* dtslot = nullptr;                // two per case
     unsigned int* status = nullptr;
+    CaseRec<double>* rec64[NTILECFG] = {};
+    CaseRec<float>* rec32[NTILECFG] = {};
+    int* tile_case[NTILECFG] = {};
+    int ntiles[NTILECFG] = {};
+    UpdChunk* chunks = nullptr;
+    int nchunks = 0;
+    // tile shape: fixed index, or -1 to time every shape on the first steps and keep the fastest
+    int tile_mode = -1;
+    int tile_cfg = 0;
+    bool tuned = false;
+    double tune_ms[NTILECFG] = {};
     // graphs
     int graph_steps = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int graph_cur = -1;
+    int graph_cur = -1, graph_cfg = -1;
     // profiling
     int profiling = 0;
     double prof[4] = {0, 0, 0, 0};
@@ -168,13 +214,19 @@ int hipfail(hc_context* c, hipError_t e, const char* where)
         if (e_ != hipSuccess) return hipfail(ctx, e_, #call);  \
     } while (0)
 
-void release_buffers(hc_context* ctx)
+void drop_graph(hc_context* ctx)
 {
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->gexec = nullptr;
     ctx->graph = nullptr;
     ctx->graph_cur = -1;
+    ctx->graph_cfg = -1;
+}
+
+void release_buffers(hc_context* ctx)
+{
+    drop_graph(ctx);
     free_dev(ctx->cvol);
     free_dev(ctx->ccx);
     free_dev(ctx->ccy);
@@ -188,6 +240,18 @@ void release_buffers(hc_context* ctx)
     free_dev(ctx->lam0);
     free_dev(ctx->bref);
     free_dev(ctx->res);
+    free_dev(ctx->maxslot);
+    free_dev(ctx->dtslot);
+    for (int k = 0; k < NTILECFG; ++k) {
+        free_dev(ctx->rec64[k]);
+        free_dev(ctx->rec32[k]);
+        free_dev(ctx->tile_case[k]);
+        ctx->ntiles[k] = 0;
+    }
+    free_dev(ctx->chunks);
+    ctx->nchunks = 0;
+    ctx->have_case = false;
+    ctx->tuned = false;
 }
 
 bool valid_ids(const hc_case_desc& d)
@@ -210,76 +274,260 @@ bool valid_ids(const hc_case_desc& d)
     return true;
 }
 
-void build_plans(hc_context* ctx)
+// the cases of one context share one kernel instantiation and one Runge-Kutta schedule
+bool same_schedule(const hc_case_desc& a, const hc_case_desc& b)
 {
-    ctx->nslots = make_rk_plans(ctx->cs.nstages, ctx->cs.alpha, ctx->cs.beta, ctx->plans);
-}
-
-template <class RF> StageArgs<RF> base_args(hc_context* ctx)
-{
-    StageArgs<RF> a{};
-    a.m.ni = ctx->ni;
-    a.m.nj = ctx->nj;
-    a.m.per_i = ctx->cs.bc[0] == HC_BC_PERIODIC;
-    a.m.per_j = ctx->cs.bc[2] == HC_BC_PERIODIC;
-    a.m.nfi = (long long)(ctx->ni + 1) * ctx->nj;
-    a.m.nfj = (long long)ctx->ni * (ctx->nj + 1);
-    a.m.cvol = ctx->cvol;
-    a.m.ccx = ctx->ccx;
-    a.m.ccy = ctx->ccy;
-    if constexpr (sizeof(RF) == 8) {
-        a.m.fi = ctx->fi;
-        a.m.fj = ctx->fj;
-    } else {
-        a.m.fi = ctx->fi32;
-        a.m.fj = ctx->fj32;
+    if (a.law != b.law || a.basis != b.basis || a.flux != b.flux || a.limiter != b.limiter) return false;
+    if (a.vscale != b.vscale || a.pscale != b.pscale || a.nstages != b.nstages || a.dt_mode != b.dt_mode) return false;
+    if (a.cfl != b.cfl) return false;
+    for (int r = 0; r < 6; ++r) {
+        if (a.beta[r] != b.beta[r]) return false;
+        for (int k = 0; k < 6; ++k)
+            if (a.alpha[r][k] != b.alpha[r][k]) return false;
     }
-    const hc_species& s = ctx->cs.sp;
-    a.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
-    a.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
-    a.vscale = ctx->cs.vscale;
-    a.pscale = ctx->cs.pscale;
-    a.limiter = ctx->cs.limiter;
-    for (int b = 0; b < 4; ++b) a.bc[b] = ctx->cs.bc[b];
-    a.dt_mode = ctx->cs.dt_mode;
-    a.N = (long long)ctx->ni * ctx->nj;
-    a.cfl = ctx->cs.cfl;
-    a.acc = ctx->acc;
-    a.tmp = ctx->tmp;
-    a.lam0 = ctx->lam0;
-    a.bref = ctx->bref;
-    a.res_out = ctx->res;
-    a.maxslot = ctx->maxslot;
-    a.dtslot = ctx->dtslot;
-    a.status = ctx->status;
-    return a;
+    return true;
 }
 
-hipError_t launch_stage(hc_context* ctx, const StageArgs<double>& a64, const StageArgs<float>& a32)
+int add_case(hc_context* ctx, int ni, int nj, const double* nodes, const hc_case_desc& desc)
+{
+    if (ni < 3 || nj < 3) return fail(ctx, HC_ERR_ARG, "mesh needs at least 3x3 cells");
+    if ((long long)ni * nj > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "mesh too large");
+    if (!valid_ids(desc)) return fail(ctx, HC_ERR_ARG, "invalid case descriptor");
+    if (!ctx->cases.empty() && !same_schedule(ctx->cases[0].cs, desc))
+        return fail(ctx, HC_ERR_ARG, "the cases of a batch must share law, basis, flux, limiter, scalings, "
+                                     "Runge-Kutta table, time-step mode and CFL");
+    CaseHost c;
+    c.ni = ni;
+    c.nj = nj;
+    c.cs = desc;
+    GeomInput gin{ni, nj, nodes, {desc.bc[0], desc.bc[1], desc.bc[2], desc.bc[3]},
+                  {desc.period_i[0], desc.period_i[1]}, {desc.period_j[0], desc.period_j[1]}};
+    if (build_geometry(gin, c.g) != 0) return fail(ctx, HC_ERR_ARG, "degenerate mesh cell");
+    c.N = (long long)ni * nj;
+    c.nfi = (long long)(ni + 1) * nj;
+    c.nfj = (long long)ni * (nj + 1);
+    c.brows = 2 * nj + 2 * ni;
+    ctx->cases.push_back(std::move(c));
+    return HC_OK;
+}
+
+template <class RF> void fill_rec(const hc_context* ctx, const CaseHost& c, int k, CaseRec<RF>& r)
+{
+    r = CaseRec<RF>{};
+    r.m.ni = c.ni;
+    r.m.nj = c.nj;
+    r.m.per_i = c.cs.bc[0] == HC_BC_PERIODIC;
+    r.m.per_j = c.cs.bc[2] == HC_BC_PERIODIC;
+    r.m.nfi = c.nfi;
+    r.m.nfj = c.nfj;
+    r.m.cvol = ctx->cvol + c.cell_off;
+    r.m.ccx = ctx->ccx + c.cell_off;
+    r.m.ccy = ctx->ccy + c.cell_off;
+    if constexpr (sizeof(RF) == 8) {
+        r.m.fi = ctx->fi + NSLOT * c.fi_off;
+        r.m.fj = ctx->fj + NSLOT * c.fj_off;
+    } else {
+        r.m.fi = ctx->fi32 + NSLOT * c.fi_off;
+        r.m.fj = ctx->fj32 + NSLOT * c.fj_off;
+    }
+    const hc_species& s = c.cs.sp;
+    r.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
+    r.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
+    for (int b = 0; b < 4; ++b) r.bc[b] = c.cs.bc[b];
+    r.N = c.N;
+    r.cell_off = c.cell_off;
+    r.bref_off = c.bref_off;
+    r.tiles_j = c.tiles_j[k];
+    r.tile0 = c.tile0[k];
+}
+
+// device layout of all cases: packed arrays, case records, tile tables, update chunks
+int build_layout(hc_context* ctx)
+{
+    release_buffers(ctx);
+    if (ctx->cases.empty()) return fail(ctx, HC_ERR_STATE, "no case");
+    ctx->cs = ctx->cases[0].cs;
+    ctx->nv = ctx->cs.law == HC_LAW_EULER ? 4 : 3;
+    ctx->nslots = make_rk_plans(ctx->cs.nstages, ctx->cs.alpha, ctx->cs.beta, ctx->plans);
+    const int nv = ctx->nv;
+    const int nc = (int)ctx->cases.size();
+    long long N = 0, nfi = 0, nfj = 0, br = 0;
+    long long nt[NTILECFG] = {};
+    const TileShape* shapes = ctx->precision == HC_PRECISION_MIXED ? TILES32 : TILES64;
+    for (auto& c : ctx->cases) {
+        c.cell_off = N;
+        c.fi_off = nfi;
+        c.fj_off = nfj;
+        c.bref_off = br;
+        N += c.N;
+        nfi += c.nfi;
+        nfj += c.nfj;
+        br += c.brows;
+        for (int k = 0; k < NTILECFG; ++k) {
+            const int ti = (c.ni + shapes[k].ti - 1) / shapes[k].ti;
+            const int tj = (c.nj + shapes[k].tj - 1) / shapes[k].tj;
+            c.tiles_j[k] = tj;
+            c.ntiles[k] = ti * tj;
+            c.tile0[k] = (int)nt[k];
+            nt[k] += (long long)ti * tj;
+        }
+    }
+    for (int k = 0; k < NTILECFG; ++k)
+        if (nt[k] > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "too many tiles");
+    ctx->Ntot = N;
+    ctx->nfi_tot = nfi;
+    ctx->nfj_tot = nfj;
+    ctx->brows_tot = br;
+    // geometry
+    HC_HIP(hipMalloc(&ctx->cvol, N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->ccx, N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->ccy, N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fi, NSLOT * nfi * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fj, NSLOT * nfj * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fi32, NSLOT * nfi * sizeof(float)));
+    HC_HIP(hipMalloc(&ctx->fj32, NSLOT * nfj * sizeof(float)));
+    for (auto& c : ctx->cases) {
+        HC_HIP(hipMemcpy(ctx->cvol + c.cell_off, c.g.cvol.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->ccx + c.cell_off, c.g.ccx.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->ccy + c.cell_off, c.g.ccy.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->fi + NSLOT * c.fi_off, c.g.fi.data(), c.g.fi.size() * sizeof(double),
+                         hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->fj + NSLOT * c.fj_off, c.g.fj.data(), c.g.fj.size() * sizeof(double),
+                         hipMemcpyHostToDevice));
+        std::vector<float> f32(c.g.fi.begin(), c.g.fi.end());
+        HC_HIP(hipMemcpy(ctx->fi32 + NSLOT * c.fi_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
+        f32.assign(c.g.fj.begin(), c.g.fj.end());
+        HC_HIP(hipMemcpy(ctx->fj32 + NSLOT * c.fj_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
+        c.g = GeomOutput{};   // the device holds it now
+    }
+    // state, accumulators, per-case slots
+    for (auto& b : ctx->q) {
+        HC_HIP(hipMalloc(&b, (size_t)nv * N * sizeof(double)));
+        HC_HIP(hipMemset(b, 0, (size_t)nv * N * sizeof(double)));
+    }
+    ctx->cur = 0;
+    if (ctx->nslots > 0) HC_HIP(hipMalloc(&ctx->acc, (size_t)ctx->nslots * nv * N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->tmp, (size_t)nv * N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->lam0, N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->res, (size_t)(nv + 1) * N * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->bref, (size_t)br * nv * sizeof(double)));
+    HC_HIP(hipMemset(ctx->bref, 0, (size_t)br * nv * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->maxslot, nc * sizeof(unsigned long long)));
+    HC_HIP(hipMalloc(&ctx->dtslot, 2 * nc * sizeof(double)));
+    {
+        std::vector<unsigned long long> m(nc, dbl_bits(DBL_MIN));
+        std::vector<double> z(2 * nc, 0.0);
+        HC_HIP(hipMemcpy(ctx->maxslot, m.data(), m.size() * sizeof(m[0]), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->dtslot, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // case records and tile tables per tile shape
+    for (int k = 0; k < NTILECFG; ++k) {
+        std::vector<CaseRec<double>> r64(nc);
+        std::vector<CaseRec<float>> r32(nc);
+        std::vector<int> tc((size_t)nt[k]);
+        for (int ci = 0; ci < nc; ++ci) {
+            const CaseHost& c = ctx->cases[ci];
+            fill_rec(ctx, c, k, r64[ci]);
+            fill_rec(ctx, c, k, r32[ci]);
+            for (int t = 0; t < c.ntiles[k]; ++t) tc[(size_t)c.tile0[k] + t] = ci;
+        }
+        HC_HIP(hipMalloc(&ctx->rec64[k], nc * sizeof(CaseRec<double>)));
+        HC_HIP(hipMalloc(&ctx->rec32[k], nc * sizeof(CaseRec<float>)));
+        HC_HIP(hipMalloc(&ctx->tile_case[k], tc.size() * sizeof(int)));
+        HC_HIP(hipMemcpy(ctx->rec64[k], r64.data(), nc * sizeof(CaseRec<double>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->rec32[k], r32.data(), nc * sizeof(CaseRec<float>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->tile_case[k], tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice));
+        ctx->ntiles[k] = (int)nt[k];
+    }
+    // 256-cell chunks of the deferred global-dt update
+    std::vector<UpdChunk> ch;
+    for (int ci = 0; ci < nc; ++ci)
+        for (long long s = 0; s < ctx->cases[ci].N; s += 256) ch.push_back(UpdChunk{ci, (int)s});
+    HC_HIP(hipMalloc(&ctx->chunks, ch.size() * sizeof(UpdChunk)));
+    HC_HIP(hipMemcpy(ctx->chunks, ch.data(), ch.size() * sizeof(UpdChunk), hipMemcpyHostToDevice));
+    ctx->nchunks = (int)ch.size();
+    ctx->have_case = true;
+    return HC_OK;
+}
+
+template <class RF> LaunchArgs<RF> launch_args(hc_context* ctx)
+{
+    LaunchArgs<RF> L{};
+    if constexpr (sizeof(RF) == 8) L.cases = ctx->rec64[ctx->tile_cfg];
+    else L.cases = ctx->rec32[ctx->tile_cfg];
+    L.tile_case = ctx->tile_case[ctx->tile_cfg];
+    L.tile_base = 0;
+    L.ntiles = ctx->ntiles[ctx->tile_cfg];
+    L.nslots = ctx->nslots;
+    L.vscale = ctx->cs.vscale;
+    L.pscale = ctx->cs.pscale;
+    L.limiter = ctx->cs.limiter;
+    L.dt_mode = ctx->cs.dt_mode;
+    L.cfl = ctx->cs.cfl;
+    L.acc = ctx->acc;
+    L.tmp = ctx->tmp;
+    L.lam0 = ctx->lam0;
+    L.bref = ctx->bref;
+    L.res_out = ctx->res;
+    L.maxslot = ctx->maxslot;
+    L.dtslot = ctx->dtslot;
+    L.status = ctx->status;
+    return L;
+}
+
+// the launch arguments differ between the precisions only in the case records
+template <class RF> LaunchArgs<RF> with_cases(const LaunchArgs<double>& L, const CaseRec<RF>* recs)
+{
+    LaunchArgs<RF> o;
+    std::memcpy(&o, &L, sizeof(L));
+    static_assert(sizeof(LaunchArgs<RF>) == sizeof(LaunchArgs<double>), "layouts differ only in pointer types");
+    o.cases = recs;
+    return o;
+}
+
+hipError_t launch_stage(hc_context* ctx, const LaunchArgs<double>& L)
 {
     const int key = ctx->cs.law * 100 + ctx->cs.basis * 10 + ctx->cs.flux;
-    const bool mixed = ctx->precision == HC_PRECISION_MIXED;
-#define HC_CASE_STAGE(L, B, F) \
-    case L * 100 + B * 10 + F: \
-        return mixed ? hc_stage_L##L##_B##B##_F##F##_P32(a32, ctx->stream) : hc_stage_L##L##_B##B##_F##F##_P64(a64, ctx->stream);
+    const int cfg = ctx->tile_cfg;
+    if (ctx->precision == HC_PRECISION_MIXED) {
+        const LaunchArgs<float> L32 = with_cases<float>(L, ctx->rec32[cfg]);
+#define HC_CASE_STAGE32(Lw, B, F) \
+    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P32(L32, cfg, ctx->stream);
+        switch (key) {
+            HC_FOR_EACH_KERNEL(HC_CASE_STAGE32)
+        default: return hipErrorInvalidValue;
+        }
+#undef HC_CASE_STAGE32
+    }
+#define HC_CASE_STAGE64(Lw, B, F) \
+    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P64(L, cfg, ctx->stream);
     switch (key) {
-        HC_FOR_EACH_KERNEL(HC_CASE_STAGE)
+        HC_FOR_EACH_KERNEL(HC_CASE_STAGE64)
     default: return hipErrorInvalidValue;
     }
-#undef HC_CASE_STAGE
+#undef HC_CASE_STAGE64
 }
 
-hipError_t launch_update(hc_context* ctx, const StageArgs<double>& a64, const StageArgs<float>& a32)
+hipError_t launch_update(hc_context* ctx, const LaunchArgs<double>& L)
 {
     const int key = ctx->cs.law * 10 + ctx->cs.basis;
-    const bool mixed = ctx->precision == HC_PRECISION_MIXED;
-#define HC_CASE_UPD(L, B) \
-    case L * 10 + B: return mixed ? hc_upd_L##L##_B##B##_P32(a32, ctx->stream) : hc_upd_L##L##_B##B##_P64(a64, ctx->stream);
+    if (ctx->precision == HC_PRECISION_MIXED) {
+        const LaunchArgs<float> L32 = with_cases<float>(L, ctx->rec32[ctx->tile_cfg]);
+#define HC_CASE_UPD32(Lw, B) \
+    case Lw * 10 + B: return hc_upd_L##Lw##_B##B##_P32(L32, ctx->chunks, ctx->nchunks, ctx->stream);
+        switch (key) {
+            HC_FOR_EACH_LAWBASIS(HC_CASE_UPD32)
+        default: return hipErrorInvalidValue;
+        }
+#undef HC_CASE_UPD32
+    }
+#define HC_CASE_UPD64(Lw, B) \
+    case Lw * 10 + B: return hc_upd_L##Lw##_B##B##_P64(L, ctx->chunks, ctx->nchunks, ctx->stream);
     switch (key) {
-        HC_FOR_EACH_LAWBASIS(HC_CASE_UPD)
+        HC_FOR_EACH_LAWBASIS(HC_CASE_UPD64)
     default: return hipErrorInvalidValue;
     }
-#undef HC_CASE_UPD
+#undef HC_CASE_UPD64
 }
 
 hipEvent_t take_event(hc_context* ctx)
@@ -321,7 +569,7 @@ void drain_events(hc_context* ctx)
     ctx->ev_pending.clear();
 }
 
-// the launches of one time step (timestepping/rungeKutta.h:97-164)
+// the launches of one time step of every case (timestepping/rungeKutta.h:97-164)
 int enqueue_step(hc_context* ctx)
 {
     const int ns = ctx->cs.nstages;
@@ -336,38 +584,31 @@ int enqueue_step(hc_context* ctx)
         bufA = ctx->q[(ctx->cur + 1) % 3];
         bufB = ctx->q[(ctx->cur + 2) % 3];
     }
-    StageArgs<double> a64 = base_args<double>(ctx);
-    StageArgs<float> a32 = base_args<float>(ctx);
+    LaunchArgs<double> L = launch_args<double>(ctx);
     const double* in = q0;
     for (int s = 0; s < ns; ++s) {
         double* out;
         if (ns == 1) out = bufA;
         else if (s == ns - 1) out = q0;   // the last stage updates the step-start state in place
         else out = (s % 2 == 0) ? bufA : bufB;
-        a64.q_in = in;
-        a64.q0 = q0;
-        a64.q_out = out;
-        a64.stage = s;
-        a64.plan = ctx->plans[s];
-        a64.reset_max = (global && ns >= 2 && s == ns - 1) ? 1 : 0;
-        a64.mode = (global && s == 0) ? MODE_GLOBAL_STAGE0 : MODE_FUSED;
-        a32.q_in = a64.q_in;
-        a32.q0 = a64.q0;
-        a32.q_out = a64.q_out;
-        a32.stage = s;
-        a32.plan = a64.plan;
-        a32.reset_max = a64.reset_max;
-        a32.mode = a64.mode;
-        hipError_t e = timed(ctx, 0, [&] { return launch_stage(ctx, a64, a32); });
+        L.q_in = in;
+        L.q0 = q0;
+        L.q_out = out;
+        L.stage = s;
+        L.plan = ctx->plans[s];
+        L.reset_max = (global && ns >= 2 && s == ns - 1) ? 1 : 0;
+        L.mode = (global && s == 0) ? MODE_GLOBAL_STAGE0 : MODE_FUSED;
+        hipError_t e = timed(ctx, 0, [&] { return launch_stage(ctx, L); });
         if (e != hipSuccess) return hipfail(ctx, e, "stage kernel launch");
         if (global && s == 0) {
-            e = timed(ctx, 1, [&] { return launch_update(ctx, a64, a32); });
+            e = timed(ctx, 1, [&] { return launch_update(ctx, L); });
             if (e != hipSuccess) return hipfail(ctx, e, "update kernel launch");
         }
         in = out;
     }
     if (global && ns == 1) {
-        hipLaunchKernelGGL(k_reset_max, dim3(1), dim3(1), 0, ctx->stream, ctx->maxslot);
+        const int nc = (int)ctx->cases.size();
+        hipLaunchKernelGGL(k_reset_max, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, ctx->maxslot, nc);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hipfail(ctx, e, "reset kernel launch");
     }
@@ -378,7 +619,16 @@ int enqueue_step(hc_context* ctx)
 int ready(hc_context* ctx)
 {
     if (!ctx) return HC_ERR_ARG;
-    if (!ctx->have_mesh || !ctx->have_case) return fail(ctx, HC_ERR_STATE, "mesh and case must be set first");
+    if (ctx->building) return fail(ctx, HC_ERR_STATE, "batch not finished (hc_batch_end)");
+    if (!ctx->have_case) return fail(ctx, HC_ERR_STATE, "mesh and case must be set first");
+    return HC_OK;
+}
+
+int case_ready(hc_context* ctx, int k)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (k < 0 || k >= (int)ctx->cases.size()) return fail(ctx, HC_ERR_ARG, "case index out of range");
     return HC_OK;
 }
 
@@ -391,11 +641,167 @@ int check_status(hc_context* ctx)
     return HC_OK;
 }
 
+int set_state(hc_context* ctx, int k, const double* q)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!q) return fail(ctx, HC_ERR_ARG, "null state");
+    CaseHost& c = ctx->cases[k];
+    const int nv = ctx->nv;
+    std::vector<double> soa((size_t)nv * c.N);
+    for (long long i = 0; i < c.N; ++i)
+        for (int v = 0; v < nv; ++v) soa[(size_t)v * c.N + i] = q[i * nv + v];
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    HC_HIP(hipMemcpy(ctx->q[ctx->cur] + nv * c.cell_off, soa.data(), soa.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+    const double z[2] = {0.0, 0.0};
+    HC_HIP(hipMemcpy(ctx->dtslot + 2 * k, z, sizeof(z), hipMemcpyHostToDevice));
+    c.have_state = true;
+    return HC_OK;
+}
+
+int get_state(hc_context* ctx, int k, double* q)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    const CaseHost& c = ctx->cases[k];
+    if (!q || !c.have_state) return fail(ctx, HC_ERR_STATE, "no state");
+    const int nv = ctx->nv;
+    std::vector<double> soa((size_t)nv * c.N);
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    HC_HIP(hipMemcpy(soa.data(), ctx->q[ctx->cur] + nv * c.cell_off, soa.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (long long i = 0; i < c.N; ++i)
+        for (int v = 0; v < nv; ++v) q[i * nv + v] = soa[(size_t)v * c.N + i];
+    return HC_OK;
+}
+
+int set_boundary_refs(hc_context* ctx, int k, const double* bref)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!bref) return fail(ctx, HC_ERR_ARG, "null boundary reference array");
+    const CaseHost& c = ctx->cases[k];
+    const size_t nb = (size_t)c.brows * ctx->nv;
+    HC_HIP(hipMemcpyAsync(ctx->bref + ctx->nv * c.bref_off, bref, nb * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+// residual of case k into ctx->res (only its tiles are launched)
+int residual_to_device(hc_context* ctx, int k)
+{
+    const CaseHost& c = ctx->cases[k];
+    if (!c.have_state) return fail(ctx, HC_ERR_STATE, "no state");
+    LaunchArgs<double> L = launch_args<double>(ctx);
+    L.mode = MODE_RESIDUAL;
+    L.q_in = ctx->q[ctx->cur];
+    L.q0 = ctx->q[ctx->cur];
+    L.q_out = ctx->q[ctx->cur];   // not written in residual mode
+    L.plan = ctx->plans[0];
+    L.tile_base = c.tile0[ctx->tile_cfg];
+    L.ntiles = c.ntiles[ctx->tile_cfg];
+    const hipError_t e = launch_stage(ctx, L);
+    if (e != hipSuccess) return hipfail(ctx, e, "residual kernel launch");
+    return HC_OK;
+}
+
+int residual(hc_context* ctx, int k, double* res)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!res) return fail(ctx, HC_ERR_ARG, "null residual array");
+    r = residual_to_device(ctx, k);
+    if (r) return r;
+    const CaseHost& c = ctx->cases[k];
+    HC_HIP(hipMemcpyAsync(res, ctx->res + (ctx->nv + 1) * c.cell_off, (size_t)(ctx->nv + 1) * c.N * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return check_status(ctx);
+}
+
+int norms(hc_context* ctx, int k, double* l2, double* linf)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!l2 || !linf) return fail(ctx, HC_ERR_ARG, "null output");
+    r = residual_to_device(ctx, k);
+    if (r) return r;
+    const CaseHost& c = ctx->cases[k];
+    hipLaunchKernelGGL(k_norm_partials, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream,
+                       ctx->res + (ctx->nv + 1) * c.cell_off, ctx->cvol + c.cell_off, c.N, ctx->nv, ctx->part);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, ctx->stream, ctx->part, NORM_BLOCKS, c.N, ctx->nout);
+    HC_HIP(hipGetLastError());
+    double out[8];
+    HC_HIP(hipMemcpyAsync(out, ctx->nout, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    for (int v = 0; v < ctx->nv; ++v) {
+        l2[v] = out[v];
+        linf[v] = out[4 + v];
+    }
+    return check_status(ctx);
+}
+
+int get_time(hc_context* ctx, int k, double* t)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!t) return fail(ctx, HC_ERR_ARG, "null output");
+    double d[2];
+    HC_HIP(hipMemcpyAsync(d, ctx->dtslot + 2 * k, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    *t = d[1];
+    return HC_OK;
+}
+
+// steps per shape and repetitions of the tile-shape trial
+constexpr int TUNE_STEPS = 1, TUNE_REPS = 2;
+
+// time every tile shape on real steps (the shapes give bit-identical results) and keep the fastest
+int autotune(hc_context* ctx)
+{
+    double best[NTILECFG];
+    for (int k = 0; k < NTILECFG; ++k) best[k] = 1e300;
+    hipEvent_t e0 = take_event(ctx), e1 = take_event(ctx);
+    for (int rep = 0; rep < TUNE_REPS; ++rep)
+        for (int k = 0; k < NTILECFG; ++k) {
+            ctx->tile_cfg = k;
+            (void)hipEventRecord(e0, ctx->stream);
+            for (int s = 0; s < TUNE_STEPS; ++s) {
+                const int r = enqueue_step(ctx);
+                if (r) return r;
+            }
+            (void)hipEventRecord(e1, ctx->stream);
+            HC_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HC_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best[k]) best[k] = ms;
+        }
+    ctx->ev_pool.push_back(e0);
+    ctx->ev_pool.push_back(e1);
+    int arg = 0;
+    for (int k = 0; k < NTILECFG; ++k) {
+        ctx->tune_ms[k] = best[k] / TUNE_STEPS;
+        if (best[k] < best[arg]) arg = k;
+    }
+    ctx->tile_cfg = arg;
+    ctx->tuned = true;
+    return HC_OK;
+}
+
+int all_states(hc_context* ctx)
+{
+    for (const auto& c : ctx->cases)
+        if (!c.have_state) return fail(ctx, HC_ERR_STATE, "no state");
+    return HC_OK;
+}
+
 }   // namespace
 
 extern "C" {
 
-const char* hc_version(void) { return "hyperboliconvergence_amd 0.1 (gfx950)"; }
+const char* hc_version(void) { return "hyperboliconvergence_amd 0.2 (gfx950)"; }
 
 int hc_create(hc_context** out, int device)
 {
@@ -405,18 +811,12 @@ int hc_create(hc_context** out, int device)
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc(&ctx->maxslot, sizeof(unsigned long long));
-    if (e == hipSuccess) e = hipMalloc(&ctx->dtslot, 2 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->status, sizeof(unsigned int));
     if (e == hipSuccess) e = hipMalloc(&ctx->part, NORM_BLOCKS * 8 * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&ctx->nout, 8 * sizeof(double));
     if (e == hipSuccess) {
-        const unsigned long long m = dbl_bits(DBL_MIN);
-        const double z[2] = {0.0, 0.0};
         const unsigned int zs = 0;
-        e = hipMemcpy(ctx->maxslot, &m, sizeof(m), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(ctx->dtslot, z, sizeof(z), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(ctx->status, &zs, sizeof(zs), hipMemcpyHostToDevice);
+        e = hipMemcpy(ctx->status, &zs, sizeof(zs), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         hc_destroy(ctx);
@@ -432,8 +832,6 @@ int hc_destroy(hc_context* ctx)
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     release_buffers(ctx);
-    free_dev(ctx->maxslot);
-    free_dev(ctx->dtslot);
     free_dev(ctx->status);
     free_dev(ctx->part);
     free_dev(ctx->nout);
@@ -457,12 +855,12 @@ int hc_set_mesh(hc_context* ctx, int ni, int nj, const double* nodes)
     HC_HIP(hipSetDevice(ctx->device));
     HC_HIP(hipStreamSynchronize(ctx->stream));
     release_buffers(ctx);
+    ctx->cases.clear();
+    ctx->building = false;
     ctx->ni = ni;
     ctx->nj = nj;
     ctx->nodes.assign(nodes, nodes + 2 * (size_t)(ni + 1) * (nj + 1));
     ctx->have_mesh = true;
-    ctx->have_case = false;
-    ctx->have_state = false;
     return HC_OK;
 }
 
@@ -473,192 +871,157 @@ int hc_set_case(hc_context* ctx, const hc_case_desc* desc)
     if (!valid_ids(*desc)) return fail(ctx, HC_ERR_ARG, "invalid case descriptor");
     HC_HIP(hipSetDevice(ctx->device));
     HC_HIP(hipStreamSynchronize(ctx->stream));
-    const int ni = ctx->ni, nj = ctx->nj;
-    GeomInput gin{ni, nj, ctx->nodes.data(), {desc->bc[0], desc->bc[1], desc->bc[2], desc->bc[3]},
-                  {desc->period_i[0], desc->period_i[1]}, {desc->period_j[0], desc->period_j[1]}};
-    GeomOutput g;
-    if (build_geometry(gin, g) != 0) return fail(ctx, HC_ERR_ARG, "degenerate mesh cell");
-    // keep the state across a case change when the variable count is unchanged
-    std::vector<double> keep;
+    // keep the state and the boundary references across a case change with the same variable count
+    std::vector<double> keep, keep_bref;
     const int nv_new = desc->law == HC_LAW_EULER ? 4 : 3;
-    const long long N = (long long)ni * nj;
-    if (ctx->have_state && ctx->nv == nv_new) {
-        keep.resize((size_t)nv_new * N);
-        HC_HIP(hipMemcpy(keep.data(), ctx->q[ctx->cur], keep.size() * sizeof(double), hipMemcpyDeviceToHost));
+    if (ctx->have_case && ctx->cases.size() == 1 && ctx->nv == nv_new) {
+        const CaseHost& c = ctx->cases[0];
+        if (c.have_state) {
+            keep.resize((size_t)nv_new * c.N);
+            HC_HIP(hipMemcpy(keep.data(), ctx->q[ctx->cur], keep.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        keep_bref.resize((size_t)c.brows * nv_new);
+        HC_HIP(hipMemcpy(keep_bref.data(), ctx->bref, keep_bref.size() * sizeof(double), hipMemcpyDeviceToHost));
     }
     release_buffers(ctx);
-    ctx->cs = *desc;
-    ctx->nv = nv_new;
-    build_plans(ctx);
-    const long long nfi = (long long)(ni + 1) * nj, nfj = (long long)ni * (nj + 1);
-    HC_HIP(hipMalloc(&ctx->cvol, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->ccx, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->ccy, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fi, NSLOT * nfi * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fj, NSLOT * nfj * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fi32, NSLOT * nfi * sizeof(float)));
-    HC_HIP(hipMalloc(&ctx->fj32, NSLOT * nfj * sizeof(float)));
-    HC_HIP(hipMemcpy(ctx->cvol, g.cvol.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HC_HIP(hipMemcpy(ctx->ccx, g.ccx.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HC_HIP(hipMemcpy(ctx->ccy, g.ccy.data(), N * sizeof(double), hipMemcpyHostToDevice));
-    HC_HIP(hipMemcpy(ctx->fi, g.fi.data(), g.fi.size() * sizeof(double), hipMemcpyHostToDevice));
-    HC_HIP(hipMemcpy(ctx->fj, g.fj.data(), g.fj.size() * sizeof(double), hipMemcpyHostToDevice));
-    {
-        std::vector<float> f32(g.fi.begin(), g.fi.end());
-        HC_HIP(hipMemcpy(ctx->fi32, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
-        f32.assign(g.fj.begin(), g.fj.end());
-        HC_HIP(hipMemcpy(ctx->fj32, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
-    }
-    const int nv = ctx->nv;
-    for (auto& b : ctx->q) HC_HIP(hipMalloc(&b, (size_t)nv * N * sizeof(double)));
-    ctx->cur = 0;
-    if (ctx->nslots > 0) HC_HIP(hipMalloc(&ctx->acc, (size_t)ctx->nslots * nv * N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->tmp, (size_t)nv * N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->lam0, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->res, (size_t)(nv + 1) * N * sizeof(double)));
-    const size_t nb = (size_t)(2 * nj + 2 * ni) * nv;
-    HC_HIP(hipMalloc(&ctx->bref, nb * sizeof(double)));
-    std::vector<double> zeros(nb, 0.0);
-    HC_HIP(hipMemcpy(ctx->bref, zeros.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+    ctx->cases.clear();
+    int r = add_case(ctx, ctx->ni, ctx->nj, ctx->nodes.data(), *desc);
+    if (r) return r;
+    r = build_layout(ctx);
+    if (r) return r;
+    if (!keep_bref.empty())
+        HC_HIP(hipMemcpy(ctx->bref, keep_bref.data(), keep_bref.size() * sizeof(double), hipMemcpyHostToDevice));
     if (!keep.empty()) {
         HC_HIP(hipMemcpy(ctx->q[0], keep.data(), keep.size() * sizeof(double), hipMemcpyHostToDevice));
-    } else {
-        ctx->have_state = false;
+        ctx->cases[0].have_state = true;
     }
-    const unsigned long long m = dbl_bits(DBL_MIN);
-    HC_HIP(hipMemcpy(ctx->maxslot, &m, sizeof(m), hipMemcpyHostToDevice));
-    ctx->have_case = true;
+    return HC_OK;
+}
+
+int hc_batch_begin(hc_context* ctx)
+{
+    if (!ctx) return HC_ERR_ARG;
+    HC_HIP(hipSetDevice(ctx->device));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    release_buffers(ctx);
+    ctx->cases.clear();
+    ctx->have_mesh = false;
+    ctx->building = true;
+    return HC_OK;
+}
+
+int hc_batch_add_case(hc_context* ctx, int ni, int nj, const double* nodes, const hc_case_desc* desc, int* case_id)
+{
+    if (!ctx || !nodes || !desc) return HC_ERR_ARG;
+    if (!ctx->building) return fail(ctx, HC_ERR_STATE, "hc_batch_begin must come first");
+    const int r = add_case(ctx, ni, nj, nodes, *desc);
+    if (r) return r;
+    if (case_id) *case_id = (int)ctx->cases.size() - 1;
+    return HC_OK;
+}
+
+int hc_batch_end(hc_context* ctx)
+{
+    if (!ctx) return HC_ERR_ARG;
+    if (!ctx->building) return fail(ctx, HC_ERR_STATE, "hc_batch_begin must come first");
+    HC_HIP(hipSetDevice(ctx->device));
+    ctx->building = false;
+    return build_layout(ctx);
+}
+
+int hc_batch_size(hc_context* ctx, int* ncases)
+{
+    if (!ctx || !ncases) return HC_ERR_ARG;
+    *ncases = (int)ctx->cases.size();
     return HC_OK;
 }
 
 int hc_set_precision(hc_context* ctx, int precision)
 {
     if (!ctx || (precision != HC_PRECISION_FP64 && precision != HC_PRECISION_MIXED)) return HC_ERR_ARG;
-    if (ctx->precision != precision) {
-        if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-        if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-        ctx->gexec = nullptr;
-        ctx->graph = nullptr;
-        ctx->graph_cur = -1;
-    }
+    if (ctx->precision == precision) return HC_OK;
+    if (ctx->building) return fail(ctx, HC_ERR_STATE, "batch not finished (hc_batch_end)");
     ctx->precision = precision;
-    return HC_OK;
-}
-
-int hc_set_boundary_refs(hc_context* ctx, const double* bref)
-{
-    int r = ready(ctx);
-    if (r) return r;
-    if (!bref) return fail(ctx, HC_ERR_ARG, "null boundary reference array");
-    const size_t nb = (size_t)(2 * ctx->nj + 2 * ctx->ni) * ctx->nv;
-    HC_HIP(hipMemcpyAsync(ctx->bref, bref, nb * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if (!ctx->have_case) return HC_OK;
+    // the tile tables depend on the precision's tile shapes: rebuild them, keeping the cases' data
+    HC_HIP(hipSetDevice(ctx->device));
     HC_HIP(hipStreamSynchronize(ctx->stream));
-    return HC_OK;
-}
-
-int hc_set_state(hc_context* ctx, const double* q)
-{
-    int r = ready(ctx);
-    if (r) return r;
-    if (!q) return fail(ctx, HC_ERR_ARG, "null state");
-    const long long N = (long long)ctx->ni * ctx->nj;
-    const int nv = ctx->nv;
-    std::vector<double> soa((size_t)nv * N);
-    for (long long c = 0; c < N; ++c)
-        for (int v = 0; v < nv; ++v) soa[(size_t)v * N + c] = q[c * nv + v];
-    HC_HIP(hipStreamSynchronize(ctx->stream));
-    HC_HIP(hipMemcpy(ctx->q[ctx->cur], soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
-    const double z[2] = {0.0, 0.0};
-    HC_HIP(hipMemcpy(ctx->dtslot, z, sizeof(z), hipMemcpyHostToDevice));
-    ctx->have_state = true;
-    return HC_OK;
-}
-
-int hc_get_state(hc_context* ctx, double* q)
-{
-    int r = ready(ctx);
-    if (r) return r;
-    if (!q || !ctx->have_state) return fail(ctx, HC_ERR_STATE, "no state");
-    const long long N = (long long)ctx->ni * ctx->nj;
-    const int nv = ctx->nv;
-    std::vector<double> soa((size_t)nv * N);
-    HC_HIP(hipStreamSynchronize(ctx->stream));
-    HC_HIP(hipMemcpy(soa.data(), ctx->q[ctx->cur], soa.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (long long c = 0; c < N; ++c)
-        for (int v = 0; v < nv; ++v) q[c * nv + v] = soa[(size_t)v * N + c];
-    return HC_OK;
-}
-
-static int residual_to_device(hc_context* ctx)
-{
-    StageArgs<double> a64 = base_args<double>(ctx);
-    StageArgs<float> a32 = base_args<float>(ctx);
-    a64.mode = a32.mode = MODE_RESIDUAL;
-    a64.q_in = a32.q_in = ctx->q[ctx->cur];
-    a64.q0 = a32.q0 = ctx->q[ctx->cur];
-    a64.plan = a32.plan = ctx->plans[0];
-    const hipError_t e = launch_stage(ctx, a64, a32);
-    if (e != hipSuccess) return hipfail(ctx, e, "residual kernel launch");
-    return HC_OK;
-}
-
-int hc_residual(hc_context* ctx, double* res)
-{
-    int r = ready(ctx);
-    if (r) return r;
-    if (!res || !ctx->have_state) return fail(ctx, HC_ERR_STATE, "no state");
-    r = residual_to_device(ctx);
-    if (r) return r;
-    const long long N = (long long)ctx->ni * ctx->nj;
-    HC_HIP(hipMemcpyAsync(res, ctx->res, (size_t)(ctx->nv + 1) * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HC_HIP(hipStreamSynchronize(ctx->stream));
-    return check_status(ctx);
-}
-
-int hc_norms(hc_context* ctx, double* l2, double* linf)
-{
-    int r = ready(ctx);
-    if (r) return r;
-    if (!l2 || !linf || !ctx->have_state) return fail(ctx, HC_ERR_STATE, "no state");
-    r = residual_to_device(ctx);
-    if (r) return r;
-    const long long N = (long long)ctx->ni * ctx->nj;
-    hipLaunchKernelGGL(k_norm_partials, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream, ctx->res, ctx->cvol, N, ctx->nv,
-                       ctx->part);
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, ctx->stream, ctx->part, NORM_BLOCKS, N, ctx->nout);
-    HC_HIP(hipGetLastError());
-    double out[8];
-    HC_HIP(hipMemcpyAsync(out, ctx->nout, sizeof(out), hipMemcpyDeviceToHost, ctx->stream));
-    HC_HIP(hipStreamSynchronize(ctx->stream));
-    for (int v = 0; v < ctx->nv; ++v) {
-        l2[v] = out[v];
-        linf[v] = out[4 + v];
+    drop_graph(ctx);
+    const TileShape* shapes = precision == HC_PRECISION_MIXED ? TILES32 : TILES64;
+    const int nc = (int)ctx->cases.size();
+    long long nt[NTILECFG] = {};
+    for (auto& c : ctx->cases)
+        for (int k = 0; k < NTILECFG; ++k) {
+            const int ti = (c.ni + shapes[k].ti - 1) / shapes[k].ti;
+            const int tj = (c.nj + shapes[k].tj - 1) / shapes[k].tj;
+            c.tiles_j[k] = tj;
+            c.ntiles[k] = ti * tj;
+            c.tile0[k] = (int)nt[k];
+            nt[k] += (long long)ti * tj;
+        }
+    for (int k = 0; k < NTILECFG; ++k) {
+        if (nt[k] > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "too many tiles");
+        std::vector<CaseRec<double>> r64(nc);
+        std::vector<CaseRec<float>> r32(nc);
+        std::vector<int> tc((size_t)nt[k]);
+        for (int ci = 0; ci < nc; ++ci) {
+            const CaseHost& c = ctx->cases[ci];
+            fill_rec(ctx, c, k, r64[ci]);
+            fill_rec(ctx, c, k, r32[ci]);
+            for (int t = 0; t < c.ntiles[k]; ++t) tc[(size_t)c.tile0[k] + t] = ci;
+        }
+        free_dev(ctx->tile_case[k]);
+        HC_HIP(hipMalloc(&ctx->tile_case[k], tc.size() * sizeof(int)));
+        HC_HIP(hipMemcpy(ctx->rec64[k], r64.data(), nc * sizeof(CaseRec<double>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->rec32[k], r32.data(), nc * sizeof(CaseRec<float>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->tile_case[k], tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice));
+        ctx->ntiles[k] = (int)nt[k];
     }
-    return check_status(ctx);
+    ctx->tuned = false;
+    ctx->tile_cfg = ctx->tile_mode >= 0 ? ctx->tile_mode : 0;
+    return HC_OK;
 }
+
+int hc_set_boundary_refs(hc_context* ctx, const double* bref) { return set_boundary_refs(ctx, 0, bref); }
+int hc_set_state(hc_context* ctx, const double* q) { return set_state(ctx, 0, q); }
+int hc_get_state(hc_context* ctx, double* q) { return get_state(ctx, 0, q); }
+int hc_residual(hc_context* ctx, double* res) { return residual(ctx, 0, res); }
+int hc_norms(hc_context* ctx, double* l2, double* linf) { return norms(ctx, 0, l2, linf); }
+int hc_get_time(hc_context* ctx, double* t) { return get_time(ctx, 0, t); }
+
+int hc_batch_set_boundary_refs(hc_context* ctx, int k, const double* bref) { return set_boundary_refs(ctx, k, bref); }
+int hc_batch_set_state(hc_context* ctx, int k, const double* q) { return set_state(ctx, k, q); }
+int hc_batch_get_state(hc_context* ctx, int k, double* q) { return get_state(ctx, k, q); }
+int hc_batch_residual(hc_context* ctx, int k, double* res) { return residual(ctx, k, res); }
+int hc_batch_norms(hc_context* ctx, int k, double* l2, double* linf) { return norms(ctx, k, l2, linf); }
+int hc_batch_get_time(hc_context* ctx, int k, double* t) { return get_time(ctx, k, t); }
 
 int hc_step(hc_context* ctx, int nsteps)
 {
     int r = ready(ctx);
     if (r) return r;
     if (nsteps < 0) return fail(ctx, HC_ERR_ARG, "negative step count");
-    if (!ctx->have_state) return fail(ctx, HC_ERR_STATE, "no state");
+    r = all_states(ctx);
+    if (r) return r;
     int done = 0;
-    int k = ctx->graph_steps;
-    // single-stage tables alternate two state buffers: capture an even number of steps
-    if (ctx->cs.nstages == 1 && (k % 2)) k += 1;
     hipEvent_t r0 = nullptr;
     if (ctx->profiling == 2 && nsteps > 0) {
         r0 = take_event(ctx);
         (void)hipEventRecord(r0, ctx->stream);
     }
+    // tile shape: fixed, or timed on the first steps of the context
+    if (ctx->tile_mode >= 0) {
+        ctx->tile_cfg = ctx->tile_mode;
+    } else if (!ctx->tuned && nsteps - done >= NTILECFG * TUNE_REPS * TUNE_STEPS && ctx->profiling != 1) {
+        r = autotune(ctx);
+        if (r) return r;
+        done += NTILECFG * TUNE_REPS * TUNE_STEPS;
+    }
+    int k = ctx->graph_steps;
+    // single-stage tables alternate two state buffers: capture an even number of steps
+    if (ctx->cs.nstages == 1 && (k % 2)) k += 1;
     if (k > 0 && ctx->profiling != 1) {
-        if (ctx->gexec && ctx->graph_cur != ctx->cur) {
-            (void)hipGraphExecDestroy(ctx->gexec);
-            (void)hipGraphDestroy(ctx->graph);
-            ctx->gexec = nullptr;
-            ctx->graph = nullptr;
-        }
+        if (ctx->gexec && (ctx->graph_cur != ctx->cur || ctx->graph_cfg != ctx->tile_cfg)) drop_graph(ctx);
         while (nsteps - done >= k) {
             if (!ctx->gexec) {
                 const int cur0 = ctx->cur;
@@ -668,11 +1031,15 @@ int hc_step(hc_context* ctx, int nsteps)
                 hipGraph_t g = nullptr;
                 const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
                 ctx->cur = cur0;
-                if (rc != HC_OK) return rc;
+                if (rc != HC_OK) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
                 if (ec != hipSuccess) return hipfail(ctx, ec, "hipStreamEndCapture");
                 ctx->graph = g;
                 HC_HIP(hipGraphInstantiate(&ctx->gexec, g, nullptr, nullptr, 0));
                 ctx->graph_cur = cur0;
+                ctx->graph_cfg = ctx->tile_cfg;
             }
             HC_HIP(hipGraphLaunch(ctx->gexec, ctx->stream));
             done += k;
@@ -685,22 +1052,29 @@ int hc_step(hc_context* ctx, int nsteps)
     if (r0) {
         hipEvent_t r1 = take_event(ctx);
         (void)hipEventRecord(r1, ctx->stream);
-        const double upd = ctx->cs.dt_mode == 0 ? 1.0 : 0.0;
+        const double upd = ctx->cs.dt_mode == HC_DT_GLOBAL ? 1.0 : 0.0;
         ctx->ev_pending.push_back({r0, r1, double(nsteps) * ctx->cs.nstages, double(nsteps) * upd, 2});
     }
     if (ctx->profiling == 1) drain_events(ctx);
     return HC_OK;
 }
 
-int hc_get_time(hc_context* ctx, double* t)
+int hc_set_tiles(hc_context* ctx, int config)
 {
-    int r = ready(ctx);
-    if (r) return r;
-    if (!t) return HC_ERR_ARG;
-    double d[2];
-    HC_HIP(hipMemcpyAsync(d, ctx->dtslot, sizeof(d), hipMemcpyDeviceToHost, ctx->stream));
-    HC_HIP(hipStreamSynchronize(ctx->stream));
-    *t = d[1];
+    if (!ctx) return HC_ERR_ARG;
+    if (config < -1 || config >= NTILECFG) return fail(ctx, HC_ERR_ARG, "tile configuration out of range");
+    ctx->tile_mode = config;
+    ctx->tuned = false;
+    ctx->tile_cfg = config >= 0 ? config : 0;
+    return HC_OK;
+}
+
+int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step)
+{
+    if (!ctx || !config) return HC_ERR_ARG;
+    *config = ctx->tile_cfg;
+    if (ms_per_step)
+        for (int k = 0; k < NTILECFG; ++k) ms_per_step[k] = ctx->tuned ? ctx->tune_ms[k] : 0.0;
     return HC_OK;
 }
 
@@ -724,11 +1098,7 @@ int hc_synchronize(hc_context* ctx)
 int hc_set_graph_steps(hc_context* ctx, int k)
 {
     if (!ctx || k < 0) return HC_ERR_ARG;
-    if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
-    if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
-    ctx->gexec = nullptr;
-    ctx->graph = nullptr;
-    ctx->graph_cur = -1;
+    if (k != ctx->graph_steps) drop_graph(ctx);
     ctx->graph_steps = k;
     return HC_OK;
 }

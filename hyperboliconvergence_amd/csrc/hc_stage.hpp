@@ -290,7 +290,8 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             fupd.n0 = -fres.n0;
             fupd.n1 = -fres.n1;
         }
-        const int type = a.bc[bid];
+        // constant indices only: the arguments may live in registers
+        const int type = bid == 0 ? a.bc[0] : bid == 1 ? a.bc[1] : bid == 2 ? a.bc[2] : a.bc[3];
         if constexpr (LAW == LAW_EULER) {
             if (type == BC_WALL) return euler_wall_flux<RF, BASIS>(a.spf, fupd, qint);
         }
@@ -423,7 +424,10 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             const double base = p.base >= 0 ? a.acc[((long long)p.base * NV + v) * a.N + c] : 0.0;
             Rt[v] = base + R[v] * p.a_ss;
         }
-        for (int u = 0; u < p.nupd; ++u) {
+        // unrolled over the plan's fixed capacity: constant indices only (the arguments may live in registers)
+#pragma unroll
+        for (int u = 0; u < 5; ++u) {
+            if (u >= p.nupd) break;
             for (int v = 0; v < NV; ++v) {
                 double* A = &a.acc[((long long)p.upd_slot[u] * NV + v) * a.N + c];
                 const double old = p.upd_first[u] ? 0.0 : *A;
@@ -498,6 +502,84 @@ template <int LAW, int BASIS> struct Stage0Update {
     }
 };
 
+// ---------------------------------------------------------------------------------------
+// Batched launch: one launch per stage covers the tiles of every case of a context.  The
+// per-case arrays are packed one case after the other (state [nv][N_c] blocks, face slots
+// [NSLOT][nf_c] blocks); a case record holds the case's mesh view with its pointers already
+// offset, its species, boundary types and offsets into the packed per-cell arrays.
+template <class RF> struct CaseRec {
+    MeshView<RF> m;
+    Species<double> sp;
+    Species<RF> spf;
+    int bc[4];
+    long long N;          // cells of the case
+    long long cell_off;   // first cell of the case in the packed per-cell arrays
+    long long bref_off;   // first boundary-reference row of the case
+    int tiles_j;          // tiles along j in the record's tile shape
+    int tile0;            // first tile of the case in the launch's tile space
+};
+
+template <class RF> struct LaunchArgs {
+    const CaseRec<RF>* cases;
+    const int* tile_case;   // case of every tile of the context's tile space
+    int tile_base;          // first tile of the launch in that space
+    int ntiles;             // tiles of the launch
+    int nslots;
+    int vscale, pscale, limiter;
+    int mode, dt_mode, stage, reset_max;
+    double cfl;
+    RkPlan plan;
+    const double* q_in;   // packed [nv][N_c] blocks
+    const double* q0;
+    double* q_out;
+    double* acc;          // packed [nslots][nv][N_c] blocks
+    double* tmp;
+    double* lam0;
+    const double* bref;   // packed [rows_c][nv] blocks
+    double* res_out;      // packed [N_c][nv+1] blocks
+    unsigned long long* maxslot;   // one per case
+    double* dtslot;                // two per case
+    unsigned int* status;
+};
+
+// the single-case view of case ci of a batched launch
+template <class RF, int NV> HC_HD StageArgs<RF> case_args(const LaunchArgs<RF>& L, const CaseRec<RF>& c, int ci)
+{
+    StageArgs<RF> a;
+    a.m = c.m;
+    a.sp = c.sp;
+    a.spf = c.spf;
+    a.vscale = L.vscale;
+    a.pscale = L.pscale;
+    a.limiter = L.limiter;
+    for (int b = 0; b < 4; ++b) a.bc[b] = c.bc[b];
+    a.mode = L.mode;
+    a.dt_mode = L.dt_mode;
+    a.stage = L.stage;
+    a.reset_max = L.reset_max;
+    a.N = c.N;
+    a.cfl = L.cfl;
+    a.q_in = L.q_in + NV * c.cell_off;
+    a.q0 = L.q0 + NV * c.cell_off;
+    a.q_out = L.q_out + NV * c.cell_off;
+    a.acc = L.nslots > 0 ? L.acc + (long long)L.nslots * NV * c.cell_off : nullptr;
+    a.tmp = L.tmp + NV * c.cell_off;
+    a.lam0 = L.lam0 + c.cell_off;
+    a.bref = L.bref + NV * c.bref_off;
+    a.res_out = L.res_out + (NV + 1) * c.cell_off;
+    a.maxslot = L.maxslot + ci;
+    a.dtslot = L.dtslot + 2 * ci;
+    a.status = L.status;
+    a.plan = L.plan;
+    return a;
+}
+
+// chunk of cells of the deferred global-dt update: 256 consecutive cells of one case
+struct UpdChunk {
+    int ci;
+    int start;
+};
+
 #if defined(__HIPCC__)
 
 // wave-level reductions for gfx950's 64-lane wavefronts
@@ -512,18 +594,27 @@ __device__ inline unsigned int wave_or(unsigned int v)
     return v;
 }
 
+// optional occupancy hint (waves per SIMD) for the register allocator, set per build
+#if defined(HC_STAGE_WAVES_PER_EU) && HC_STAGE_WAVES_PER_EU > 0
+#define HC_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(HC_STAGE_WAVES_PER_EU)))
+#else
+#define HC_STAGE_OCCUPANCY
+#endif
+
 template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR>
-__global__ void __launch_bounds__(NTHR) k_stage(StageArgs<RF> a, int tiles_j, int ntiles)
+__global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF> L)
 {
     using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
     // 8 XCDs, so give each XCD a contiguous band of tiles (shared halos stay in one L2)
-    const int b = blockIdx.x;
-    const int t = xcd_tile(b, ntiles);
-    const int ti = t / tiles_j, tj = t % tiles_j;
-    const int i0 = ti * TI, j0 = tj * TJ;
+    const int t = L.tile_base + xcd_tile(blockIdx.x, L.ntiles);
+    const int ci = L.tile_case[t];
+    const CaseRec<RF>& c = L.cases[ci];
+    const StageArgs<RF> a = case_args<RF, T::NV>(L, c, ci);
+    const int lt = t - c.tile0;
+    const int i0 = (lt / c.tiles_j) * TI, j0 = (lt % c.tiles_j) * TJ;
     unsigned int flag = 0;
     T::load(a, s, i0, j0, tid);
     __syncthreads();
@@ -539,13 +630,16 @@ __global__ void __launch_bounds__(NTHR) k_stage(StageArgs<RF> a, int tiles_j, in
         if (a.mode == MODE_GLOBAL_STAGE0) atomicMax(a.maxslot, dbl_bits(m));
         if (f) atomicOr(a.status, f);
     }
-    if (a.reset_max && b == 0 && tid == 0) *a.maxslot = dbl_bits(DBL_MIN);
+    // the last stage re-arms the case's max slot for the next step (no stage reads it then)
+    if (a.reset_max && lt == 0 && tid == 0) *a.maxslot = dbl_bits(DBL_MIN);
 }
 
 template <int LAW, int BASIS, class RF>
-__global__ void __launch_bounds__(256) k_stage0_update(StageArgs<RF> a)
+__global__ void __launch_bounds__(256) k_stage0_update(LaunchArgs<RF> L, const UpdChunk* chunks)
 {
-    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const UpdChunk ch = chunks[blockIdx.x];
+    const StageArgs<RF> a = case_args<RF, LawTraits<LAW>::nv>(L, L.cases[ch.ci], ch.ci);
+    const long long c = (long long)ch.start + threadIdx.x;
     unsigned int flag = 0;
     Stage0Update<LAW, BASIS>::run(a, c, flag);
     const unsigned int f = wave_or(flag);

@@ -168,6 +168,38 @@ int hc_device_state(hc_context* ctx, double** dptr);
  */
 int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps);
 
+/*
+ * Batches.  A context can hold several cases (meshes with their own boundary types, periodic
+ * translations, species and boundary references) that share the law, basis, flux, limiter,
+ * scalings, Runge-Kutta table, time-step mode and CFL: every stage launch then covers the
+ * tiles of all cases (one tile table), and hc_step advances all of them.  The reference runs
+ * one integrate() per case (timestepping/rungeKutta.h:44-167); these entry points have no
+ * reference counterpart.  Sequence: hc_batch_begin, hc_batch_add_case per case (geometry is
+ * built here), hc_batch_end (device layout), then per case hc_batch_set_boundary_refs and
+ * hc_batch_set_state.  The single-case entry points above act on case 0.
+ */
+int hc_batch_begin(hc_context* ctx);
+int hc_batch_add_case(hc_context* ctx, int ni, int nj, const double* nodes, const hc_case_desc* desc,
+                      int* case_id);
+int hc_batch_end(hc_context* ctx);
+int hc_batch_size(hc_context* ctx, int* ncases);
+int hc_batch_set_boundary_refs(hc_context* ctx, int case_id, const double* bref);
+int hc_batch_set_state(hc_context* ctx, int case_id, const double* q);
+int hc_batch_get_state(hc_context* ctx, int case_id, double* q);
+int hc_batch_residual(hc_context* ctx, int case_id, double* res);
+int hc_batch_norms(hc_context* ctx, int case_id, double* l2, double* linf);
+int hc_batch_get_time(hc_context* ctx, int case_id, double* t);
+
+/*
+ * Tile shape of the stage kernels: config -1 (default) times every shape on the first steps of
+ * the context (all shapes give bit-identical results) and keeps the fastest; 0 or 1 fixes one.
+ * hc_get_tiles returns the shape in use and, after timing, the milliseconds per step of each
+ * shape (HC_NTILECFG values).  No counterpart in the reference.
+ */
+#define HC_NTILECFG 2
+int hc_set_tiles(hc_context* ctx, int config);
+int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step);
+
 /* Library version string. */
 const char* hc_version(void);
 

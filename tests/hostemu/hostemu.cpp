@@ -138,11 +138,21 @@ template <class RF> void dispatch_update(const hc_case_desc& cs, const StageArgs
     else run_update<1, 1, RF>(a);
 }
 
-// tile shapes of hc_launch.hpp: FP64 4x32, mixed 8x32
+// tile shape of the emulated launch (hc_tiles.hpp), selected with emu_set_tile_config
+int g_tile_cfg = 0;
+
+template <int K> void stage_shape(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
+{
+    constexpr TileShape S64 = TILES64[K], S32 = TILES32[K];
+    if (e.precision == HC_PRECISION_MIXED) dispatch_stage<float, S32.ti, S32.tj, S32.nt>(e.cs, a32);
+    else dispatch_stage<double, S64.ti, S64.tj, S64.nt>(e.cs, a64);
+}
+
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
-    if (e.precision == HC_PRECISION_MIXED) dispatch_stage<float, TI32, TJ32, NT32>(e.cs, a32);
-    else dispatch_stage<double, TI64, TJ64, NT64>(e.cs, a64);
+    static_assert(NTILECFG == 2, "one stage_shape per tile shape");
+    if (g_tile_cfg == 1) stage_shape<1>(e, a64, a32);
+    else stage_shape<0>(e, a64, a32);
 }
 
 void update(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
@@ -181,6 +191,13 @@ int setup(Emu& e, const hc_case_desc* cs, int ni, int nj, const double* nodes, c
 }   // namespace
 
 extern "C" {
+
+int emu_set_tile_config(int cfg)
+{
+    if (cfg < 0 || cfg >= NTILECFG) return -1;
+    g_tile_cfg = cfg;
+    return 0;
+}
 
 int emu_residual(const hc_case_desc* cs, int ni, int nj, const double* nodes, const double* q, const double* bref,
                  int precision, double* res, unsigned int* status)

@@ -19,7 +19,13 @@ def lib():
         _L.emu_residual.argtypes = [P, i, i, P, P, P, i, P, P]
         _L.emu_integrate.argtypes = [P, i, i, P, P, P, i, i, P, P]
         _L.emu_geometry.argtypes = [P, i, i, P, P, P, P, P]
+        _L.emu_set_tile_config.argtypes = [i]
     return _L
+
+
+def set_tile_config(cfg):
+    """Tile shape of the emulated launches (index into the kernel's shape table)."""
+    assert lib().emu_set_tile_config(int(cfg)) == 0
 
 
 def _p(a):

@@ -1,0 +1,146 @@
+"""GPU: tile shapes, the tile-shape autotuner and batched contexts, through the C-ABI.
+
+Every tile shape must give the oracle's results bit for bit (the shape changes which lanes
+compute a face, never the arithmetic), so the autotuner may switch shapes freely; a batch of
+cases stepped by one launch per stage must give, case by case, exactly the results of a
+context holding that case alone (and hence of the oracle).
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def _cyl(nth, nr, mach, alpha, flux=None, perturb=0.0, seed=3):
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    cfg = cases.config_cylinder(nth, nr, mach, alpha, flux=hca.FLUX_ROE if flux is None else flux)
+    if perturb:
+        rng = np.random.default_rng(seed)
+        cfg["q"] = cfg["q"] * (1.0 + perturb * rng.standard_normal(cfg["q"].shape))
+    return cfg
+
+
+def _single(cfg, precision=0, tiles=None):
+    import hyperboliconvergence_amd as hca
+    ctx = hca.Context(0)
+    ctx.set_mesh(cfg["nodes"])
+    ctx.set_case(cfg["case"])
+    ctx.set_precision(precision)
+    if tiles is not None:
+        ctx.set_tiles(tiles)
+    ctx.set_boundary_refs(cfg["bref"])
+    ctx.set_state(cfg["q"])
+    return ctx
+
+
+def _batch(cfgs, precision=0):
+    import hyperboliconvergence_amd as hca
+    b = hca.Batch(0)
+    b.begin()
+    for cfg in cfgs:
+        b.add_case(cfg["nodes"], cfg["case"])
+    b.end()
+    b.set_precision(precision)
+    for k, cfg in enumerate(cfgs):
+        b.set_case_boundary_refs(k, cfg["bref"])
+        b.set_case_state(k, cfg["q"])
+    return b
+
+
+@pytest.mark.parametrize("tiles", [0, 1])
+def test_every_tile_shape_bit_identical_to_oracle(tiles):
+    cfg = _cyl(40, 21, 0.3, 2.0, perturb=0.01)   # ragged in both directions for both shapes
+    with _single(cfg, tiles=tiles) as ctx:
+        r = ctx.residual()
+        ctx.step(6)
+        q = ctx.get_state()
+        assert ctx.tiles()[0] == tiles
+        assert ctx.status() == 0
+    assert np.array_equal(r, po.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"]))
+    qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 6, cfg["bref"])
+    assert np.array_equal(q, qo)
+
+
+def test_autotuner_times_both_shapes_and_keeps_results():
+    cfg = _cyl(64, 32, 0.4, -3.0, perturb=0.01)
+    with _single(cfg, tiles=-1) as ctx:
+        ctx.set_graph_steps(4)
+        ctx.step(12)   # 4 trial steps, then two graphs of 4
+        q = ctx.get_state()
+        chosen, ms = ctx.tiles()
+    print("tile shape", chosen, "ms/step", ms)
+    assert chosen in (0, 1) and np.all(ms > 0)
+    assert ms[chosen] == ms.min()
+    qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 12, cfg["bref"])
+    assert np.array_equal(q, qo)
+
+
+def _mixed_batch_cfgs():
+    # different sizes, Mach numbers and angles of attack; perturbed so every face is active
+    return [_cyl(64, 32, 0.3, 0.0, perturb=0.01, seed=1), _cyl(40, 21, 0.55, 4.0, perturb=0.01, seed=2),
+            _cyl(128, 64, 0.25, -2.5, perturb=0.01, seed=3), _cyl(32, 16, 0.7, 1.0, perturb=0.01, seed=4)]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_equals_single_case_contexts(precision):
+    cfgs = _mixed_batch_cfgs()
+    with _batch(cfgs, precision) as b:
+        assert len(b) == 4
+        res = [b.case_residual(k) for k in range(4)]
+        b.set_graph_steps(3)
+        b.step(10)
+        qb = [b.get_case_state(k) for k in range(4)]
+        assert b.status() == 0
+    for k, cfg in enumerate(cfgs):
+        with _single(cfg, precision) as ctx:
+            r = ctx.residual()
+            ctx.step(10)
+            q = ctx.get_state()
+        assert np.array_equal(res[k], r), k
+        assert np.array_equal(qb[k], q), k
+
+
+def test_batch_fp64_matches_oracle_per_case():
+    cfgs = _mixed_batch_cfgs()[:3]
+    with _batch(cfgs) as b:
+        b.step(5)
+        qb = [b.get_case_state(k) for k in range(3)]
+        norms = [b.case_norms(k) for k in range(3)]
+    for k, cfg in enumerate(cfgs):
+        qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 5, cfg["bref"])
+        assert np.array_equal(qb[k], qo), k
+        ro = po.residual(cfg["case"], cfg["nodes"], qo, cfg["bref"])
+        l2o, linfo = po.norms(po.dual(cfg["nodes"]), ro)
+        assert np.allclose(norms[k][0], l2o, rtol=1e-12, atol=0), k
+        assert np.array_equal(norms[k][1], linfo), k
+
+
+def test_batch_global_time_step_is_per_case():
+    """Global dt (SSP-RK3): each case reduces its own max(lambda/V) and keeps its own time."""
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    cfgs = [cases.config_c3(n, flux=hca.FLUX_AUSMPUP) for n in (24, 40, 33)]
+    with _batch(cfgs) as b:
+        b.step(4)
+        qb = [b.get_case_state(k) for k in range(3)]
+        tb = [b.case_time(k) for k in range(3)]
+    for k, cfg in enumerate(cfgs):
+        qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 4, cfg["bref"])
+        assert np.array_equal(qb[k], qo), k
+        assert tb[k] == to, k
+    assert len(set(tb)) == 3
+
+
+def test_batch_rejects_cases_with_different_schedules():
+    import hyperboliconvergence_amd as hca
+    a = _cyl(32, 16, 0.3, 0.0)
+    c = _cyl(32, 16, 0.3, 0.0, flux=hca.FLUX_RUSANOV)
+    b = hca.Batch(0)
+    b.begin()
+    b.add_case(a["nodes"], a["case"])
+    with pytest.raises(hca.HCError):
+        b.add_case(c["nodes"], c["case"])
+    b.close()

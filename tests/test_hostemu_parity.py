@@ -15,6 +15,14 @@ from conftest import rel_err
 G = 1.4
 
 
+@pytest.fixture(autouse=True, params=[0, 1], ids=["tiles0", "tiles1"])
+def tile_config(request):
+    """Every test runs with each tile shape of the kernels (the results must not depend on it)."""
+    pyemu.set_tile_config(request.param)
+    yield request.param
+    pyemu.set_tile_config(0)
+
+
 def _ogrid(nr, nth, basis, flux, limiter, bc=(po.BC_WALL, po.BC_RIEMANN, 0, 0), perturb=0.02, rk=44, dt=po.DT_LOCAL):
     from hyperboliconvergence_amd import cases
     nodes = po.mesh_ogrid(nr, nth, 0.5, 20.0, 1e-3)
