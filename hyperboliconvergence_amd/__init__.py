@@ -20,7 +20,7 @@ SCALE_CONVECTIVE, SCALE_ACOUSTIC, SCALE_ADAPTIVE = range(3)
 (LIM_NOLIMIT1, LIM_NOLIMIT2, LIM_NOLIMIT3, LIM_VANALBADA2, LIM_MINMOD2, LIM_SUPERBEE2,
  LIM_MC2, LIM_VANLEER2, LIM_CADA3, LIM_SMOOTHVANALBADA) = range(10)
 BC_PERIODIC, BC_RIEMANN, BC_ENTROPY, BC_FIXED, BC_WALL = range(5)
-DT_GLOBAL, DT_LOCAL = 0, 1
+DT_GLOBAL, DT_LOCAL, DT_FIXED = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 STATUS_NONFINITE, STATUS_NONPHYSICAL, STATUS_BCNORMAL = 1, 2, 4
 

@@ -269,7 +269,7 @@ bool valid_ids(const hc_case_desc& d)
     if ((d.bc[0] == HC_BC_PERIODIC) != (d.bc[1] == HC_BC_PERIODIC)) return false;
     if ((d.bc[2] == HC_BC_PERIODIC) != (d.bc[3] == HC_BC_PERIODIC)) return false;
     if (d.nstages < 1 || d.nstages > 6) return false;
-    if (d.dt_mode < 0 || d.dt_mode > 1) return false;
+    if (d.dt_mode < 0 || d.dt_mode > 2) return false;
     if (!(d.cfl > 0.0)) return false;
     return true;
 }

@@ -445,6 +445,8 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         if (a.dt_mode == 0) {
             const double dt = a.dtslot[0];
             fac = (p.beta * dt) / vol;
+        } else if (a.dt_mode == 2) {
+            fac = (p.beta * a.cfl) / vol;   // fixed step: cfl holds dt
         } else {
             double l0;
             if (a.stage == 0) {
@@ -632,6 +634,11 @@ __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF
     }
     // the last stage re-arms the case's max slot for the next step (no stage reads it then)
     if (a.reset_max && lt == 0 && tid == 0) *a.maxslot = dbl_bits(DBL_MIN);
+    // fixed step: the case's time advances once per step (one thread of the case's first tile)
+    if (a.dt_mode == 2 && a.mode == MODE_FUSED && a.stage == 0 && lt == 0 && tid == 0) {
+        a.dtslot[0] = a.cfl;
+        a.dtslot[1] = a.dtslot[1] + a.cfl;
+    }
 }
 
 template <int LAW, int BASIS, class RF>

@@ -45,7 +45,9 @@ enum { HC_LIM_NOLIMIT1 = 0, HC_LIM_NOLIMIT2 = 1, HC_LIM_NOLIMIT3 = 2, HC_LIM_VAN
        HC_LIM_MINMOD2 = 4, HC_LIM_SUPERBEE2 = 5, HC_LIM_MC2 = 6, HC_LIM_VANLEER2 = 7,
        HC_LIM_CADA3 = 8, HC_LIM_SMOOTHVANALBADA = 9 };
 enum { HC_BC_PERIODIC = 0, HC_BC_RIEMANN = 1, HC_BC_ENTROPY = 2, HC_BC_FIXED = 3, HC_BC_WALL = 4 };
-enum { HC_DT_GLOBAL = 0, HC_DT_LOCAL = 1 };
+/* HC_DT_FIXED: hc_case_desc.cfl holds the time step itself; no spectral-radius reduction
+   (the differentiable form of the time loop, used by the adjoint) */
+enum { HC_DT_GLOBAL = 0, HC_DT_LOCAL = 1, HC_DT_FIXED = 2 };
 enum { HC_PRECISION_FP64 = 0, HC_PRECISION_MIXED = 1 };
 
 /* status bits (hc_get_status) */

@@ -1276,6 +1276,7 @@ bool valid_case(const orc_case& cs, int ni, int nj)
     if (cs.law == ORC_LAW_SCALAR && cs.flux > ORC_FLUX_RUSANOV) return false;
     if (cs.flux < 0 || cs.flux > ORC_FLUX_ROE_FACEFRAME) return false;
     if (cs.nstages < 1 || cs.nstages > 6) return false;
+    if (cs.dt_mode < ORC_DT_GLOBAL || cs.dt_mode > ORC_DT_FIXED) return false;
     return true;
 }
 
@@ -1587,6 +1588,7 @@ int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, doubl
             residual_calc(F, q1.data(), resStage[stg]);
             lap(2, t0);
             if (stg == 0 && cs->dt_mode == ORC_DT_GLOBAL) dt = cs->cfl / spectral_radius_field(F.mesh, resStage[0]);
+            if (stg == 0 && cs->dt_mode == ORC_DT_FIXED) dt = cs->cfl;
             lap(3, t0);
             // rungeKuttaAccumulation, rungeKutta.h:176-197
             const std::vector<FluxRes>* rs = resStage.data();
@@ -1599,11 +1601,11 @@ int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, doubl
                     for (int v = 0; v < nv; ++v) rt[v] += rs[k][c].f[v] * alpha[k];
             }
             lap(4, t0);
-            const double stepfac = cs->dt_mode == ORC_DT_GLOBAL ? cs->beta[stg] * dt : cs->beta[stg] * cs->cfl;
+            const bool by_vol = cs->dt_mode != ORC_DT_LOCAL;   // global and fixed: (beta dt) / V
+            const double stepfac = by_vol ? cs->beta[stg] * dt : cs->beta[stg] * cs->cfl;
 #pragma omp parallel for schedule(static)
             for (long c = 0; c < (long)N; ++c) {
-                const double fac = cs->dt_mode == ORC_DT_GLOBAL ? stepfac / F.mesh.cells[c].vol
-                                                                : stepfac / rs[0][c].lambda;
+                const double fac = by_vol ? stepfac / F.mesh.cells[c].vol : stepfac / rs[0][c].lambda;
                 double qc[MAXV];
                 sol2cons(*cs, F.sp, &q0[(size_t)c * nv], qc);
                 for (int v = 0; v < nv; ++v) qc[v] = qc[v] + resTotal[(size_t)c * nv + v] * fac;

@@ -41,7 +41,8 @@ enum { ORC_LIM_NOLIMIT1 = 0, ORC_LIM_NOLIMIT2 = 1, ORC_LIM_NOLIMIT3 = 2,
        ORC_LIM_SMOOTHVANALBADA = 9 };
 enum { ORC_BC_PERIODIC = 0, ORC_BC_RIEMANN = 1, ORC_BC_ENTROPY = 2,
        ORC_BC_FIXED = 3, ORC_BC_WALL = 4 };
-enum { ORC_DT_GLOBAL = 0, ORC_DT_LOCAL = 1 };
+/* ORC_DT_FIXED: cfl holds the time step itself (no spectral-radius reduction) */
+enum { ORC_DT_GLOBAL = 0, ORC_DT_LOCAL = 1, ORC_DT_FIXED = 2 };
 
 typedef struct {
     int law, basis, flux, vscale, pscale, limiter;

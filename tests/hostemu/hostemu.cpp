@@ -246,6 +246,10 @@ int emu_integrate(const hc_case_desc* cs, int ni, int nj, const double* nodes, d
             in = out;
         }
         if (global && ns == 1) e.maxslot = dbl_bits(DBL_MIN);
+        if (cs->dt_mode == HC_DT_FIXED) {
+            e.dtslot[0] = cs->cfl;
+            e.dtslot[1] = e.dtslot[1] + cs->cfl;
+        }
         if (ns == 1) e.cur = 1 - e.cur;
     }
     const long long N = (long long)ni * nj;

@@ -20,6 +20,8 @@ def lib():
         _L.emu_integrate.argtypes = [P, i, i, P, P, P, i, i, P, P]
         _L.emu_geometry.argtypes = [P, i, i, P, P, P, P, P]
         _L.emu_set_tile_config.argtypes = [i]
+        _L.emu_tangent_residual.argtypes = [P, i, i, P, P, P, P, P, C.c_double, P, P]
+        _L.emu_adjoint_residual.argtypes = [P, i, i, P, P, P, P, P, P, P]
     return _L
 
 
@@ -65,3 +67,32 @@ def geometry(cs, nodes):
                             _p(X), _p(fi), _p(fj))
     assert rc == 0
     return cells, X, fi, fj
+
+
+def tangent_residual(cs, nodes, q, bref, qd, brefd=None, minfd=0.0):
+    """(R(q) [ni][nj][nv+1], Rdot [ni][nj][nv]) of the tangent kernels."""
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    nv = 4 if cs.law == 1 else 3
+    res = np.zeros((ni, nj, nv + 1))
+    resd = np.zeros((ni, nj, nv))
+    bd = None if brefd is None else _p(np.ascontiguousarray(brefd, float))
+    rc = lib().emu_tangent_residual(C.cast(C.pointer(cs), C.c_void_p), ni, nj, _p(np.ascontiguousarray(nodes)),
+                                    _p(np.ascontiguousarray(q, float)), _p(np.ascontiguousarray(qd, float)),
+                                    _p(np.ascontiguousarray(bref, float)), bd, float(minfd), _p(res), _p(resd))
+    assert rc == 0, rc
+    return res, resd
+
+
+def adjoint_residual(cs, nodes, q, bref, ybar):
+    """(qbar [ni][nj][nv], brefbar [rows][nv], minfbar) of the adjoint kernels."""
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    nv = 4 if cs.law == 1 else 3
+    qbar = np.zeros((ni, nj, nv))
+    brefbar = np.zeros_like(np.ascontiguousarray(bref, float))
+    mb = C.c_double()
+    rc = lib().emu_adjoint_residual(C.cast(C.pointer(cs), C.c_void_p), ni, nj, _p(np.ascontiguousarray(nodes)),
+                                    _p(np.ascontiguousarray(q, float)), _p(np.ascontiguousarray(bref, float)),
+                                    _p(np.ascontiguousarray(ybar, float)), _p(qbar), _p(brefbar), C.byref(mb))
+    assert rc == 0, rc
+    return qbar, brefbar, mb.value
+

@@ -1,16 +1,22 @@
 """GPU: tile shapes, the tile-shape autotuner and batched contexts, through the C-ABI.
 
-Every tile shape must give the oracle's results bit for bit (the shape changes which lanes
+Every tile shape must give the same results bit for bit (the shape changes which lanes
 compute a face, never the arithmetic), so the autotuner may switch shapes freely; a batch of
 cases stepped by one launch per stage must give, case by case, exactly the results of a
-context holding that case alone (and hence of the oracle).
+context holding that case alone.  Against the oracle the bars are the parity tolerances:
+pow() in the far-field boundary is not correctly rounded on either side, so GPU and host
+may differ in the last bit there and the difference grows slowly along a trajectory.
 """
 import numpy as np
 import pytest
 
 import pyoracle as po
+from conftest import rel_err
 
 pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-12
+TRAJ_TOL = 1e-10
 
 
 def _cyl(nth, nr, mach, alpha, flux=None, perturb=0.0, seed=3):
@@ -50,18 +56,21 @@ def _batch(cfgs, precision=0):
     return b
 
 
-@pytest.mark.parametrize("tiles", [0, 1])
-def test_every_tile_shape_bit_identical_to_oracle(tiles):
+def test_every_tile_shape_gives_identical_results():
     cfg = _cyl(40, 21, 0.3, 2.0, perturb=0.01)   # ragged in both directions for both shapes
-    with _single(cfg, tiles=tiles) as ctx:
-        r = ctx.residual()
-        ctx.step(6)
-        q = ctx.get_state()
-        assert ctx.tiles()[0] == tiles
-        assert ctx.status() == 0
-    assert np.array_equal(r, po.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"]))
+    out = []
+    for tiles in (0, 1):
+        with _single(cfg, tiles=tiles) as ctx:
+            r = ctx.residual()
+            ctx.step(6)
+            out.append((r, ctx.get_state()))
+            assert ctx.tiles()[0] == tiles
+            assert ctx.status() == 0
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    ro = po.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"])
     qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 6, cfg["bref"])
-    assert np.array_equal(q, qo)
+    assert rel_err(out[0][0], ro) <= RES_TOL
+    assert rel_err(out[0][1], qo) <= TRAJ_TOL
 
 
 def test_autotuner_times_both_shapes_and_keeps_results():
@@ -74,8 +83,12 @@ def test_autotuner_times_both_shapes_and_keeps_results():
     print("tile shape", chosen, "ms/step", ms)
     assert chosen in (0, 1) and np.all(ms > 0)
     assert ms[chosen] == ms.min()
+    with _single(cfg, tiles=0) as ctx:
+        ctx.step(12)
+        q0 = ctx.get_state()
+    assert np.array_equal(q, q0)
     qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 12, cfg["bref"])
-    assert np.array_equal(q, qo)
+    assert rel_err(q, qo) <= TRAJ_TOL
 
 
 def _mixed_batch_cfgs():
@@ -111,11 +124,11 @@ def test_batch_fp64_matches_oracle_per_case():
         norms = [b.case_norms(k) for k in range(3)]
     for k, cfg in enumerate(cfgs):
         qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 5, cfg["bref"])
-        assert np.array_equal(qb[k], qo), k
-        ro = po.residual(cfg["case"], cfg["nodes"], qo, cfg["bref"])
+        assert rel_err(qb[k], qo) <= TRAJ_TOL, k
+        ro = po.residual(cfg["case"], cfg["nodes"], qb[k], cfg["bref"])
         l2o, linfo = po.norms(po.dual(cfg["nodes"]), ro)
-        assert np.allclose(norms[k][0], l2o, rtol=1e-12, atol=0), k
-        assert np.array_equal(norms[k][1], linfo), k
+        assert np.allclose(norms[k][0], l2o, rtol=1e-10, atol=0), k
+        assert np.allclose(norms[k][1], linfo, rtol=1e-10, atol=0), k
 
 
 def test_batch_global_time_step_is_per_case():
@@ -129,9 +142,13 @@ def test_batch_global_time_step_is_per_case():
         tb = [b.case_time(k) for k in range(3)]
     for k, cfg in enumerate(cfgs):
         qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 4, cfg["bref"])
-        assert np.array_equal(qb[k], qo), k
-        assert tb[k] == to, k
+        assert rel_err(qb[k], qo) <= TRAJ_TOL, k
+        assert abs(tb[k] - to) <= 1e-12 * to, k
     assert len(set(tb)) == 3
+    for k, cfg in enumerate(cfgs):
+        with _single(cfg) as ctx:
+            ctx.step(4)
+            assert np.array_equal(ctx.get_state(), qb[k]) and ctx.time() == tb[k], k
 
 
 def test_batch_rejects_cases_with_different_schedules():

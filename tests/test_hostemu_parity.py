@@ -77,9 +77,11 @@ def test_boundary_kinds_bit_identical(bc):
 
 
 @pytest.mark.parametrize("rk", [11, 22, 33, 34, 44])
-@pytest.mark.parametrize("dt", [po.DT_GLOBAL, po.DT_LOCAL])
+@pytest.mark.parametrize("dt", [po.DT_GLOBAL, po.DT_LOCAL, po.DT_FIXED])
 def test_rk_tables_bit_identical(rk, dt):
     cs, nodes, q, bref = _ogrid(8, 16, 1, po.FLUX_AUSMPUP, po.LIM_VANALBADA2, rk=rk, dt=dt)
+    if dt == po.DT_FIXED:
+        cs.cfl = 1e-4   # the time step itself
     qe, te, _ = pyemu.integrate(cs, nodes, q, bref, 3)
     qo, to, _ = po.integrate(cs, nodes, q, 3, bref)
     assert np.array_equal(qe, qo) and te == to

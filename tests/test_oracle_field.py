@@ -253,6 +253,30 @@ def test_local_time_stepping_uses_stage0_lambda():
     assert np.allclose(q1, qc + res[..., :4] * (0.7 / res[..., 4:5]), rtol=1e-14, atol=1e-15)
 
 
+def test_fixed_time_step_is_a_hand_update():
+    """DT_FIXED: cfl holds dt; one SSP-RK2 step equals the hand-written two-stage update with
+    (beta dt)/V, and the time advances by dt per step."""
+    nodes = po.mesh_ogrid(6, 12, 0.5, 20.0, 1e-2)
+    qinf = np.array([0.3, 0.0, 1.0, 1.0 / G])
+    rng = np.random.default_rng(14)
+    from hyperboliconvergence_amd import cases
+    qc = cases.prim_to_cons(qinf * (1 + 0.02 * rng.standard_normal((6, 12, 4))))
+    bc = cases.prim_to_cons(np.broadcast_to(qinf, (po.bref_size(6, 12), 4)).copy())
+    dt = 3e-3
+    cs = po.make_case(law=po.LAW_EULER, basis=po.BASIS_CONSERVED, flux=po.FLUX_AUSMPUP,
+                      limiter=po.LIM_SMOOTHVANALBADA, bc=(po.BC_WALL, po.BC_RIEMANN, 0, 0), rk=22, cfl=dt,
+                      dt_mode=po.DT_FIXED, minf=0.3)
+    vol = po.dual(nodes)[..., 0:1]
+    r0 = po.residual(cs, nodes, qc, bc)[..., :4]
+    q1 = qc + r0 * ((1.0 * dt) / vol)
+    r1 = po.residual(cs, nodes, q1, bc)[..., :4]
+    q2 = qc + (r0 * 0.5 + r1 * 0.5) * ((1.0 * dt) / vol)
+    qo, t, _ = po.integrate(cs, nodes, qc, 1, bc)
+    assert np.allclose(qo, q2, rtol=1e-14, atol=1e-15)
+    _, t3, _ = po.integrate(cs, nodes, qc, 3, bc)
+    assert t == dt and t3 == (dt + dt) + dt
+
+
 @pytest.mark.parametrize("flux", [po.FLUX_AUSMPUP, po.FLUX_SLAU, po.FLUX_ROE])
 def test_subsonic_isentropic_vortex_converges(flux):
     """Shu's isentropic vortex (closed-form solution), strength 2 so that every face stays subsonic,
