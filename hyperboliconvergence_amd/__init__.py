@@ -32,6 +32,7 @@ C_ABI_SYMBOLS = (
     "hc_batch_begin", "hc_batch_add_case", "hc_batch_end", "hc_batch_size", "hc_batch_set_boundary_refs",
     "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
     "hc_set_tiles", "hc_get_tiles",
+    "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
 )
 NTILECFG = 2
 
@@ -100,6 +101,11 @@ def library():
         "hc_batch_get_time": (i, [P, i, P]),
         "hc_set_tiles": (i, [P, i]),
         "hc_get_tiles": (i, [P, P, P]),
+        "hc_tangent_residual": (i, [P, P, P, d, P, P]),
+        "hc_adjoint_residual": (i, [P, P, P, P, P]),
+        "hc_adjoint_forward": (i, [P, i, i]),
+        "hc_adjoint_backward": (i, [P, P, P, P, P]),
+        "hc_tangent_steps": (i, [P, i, P, P, d, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -223,6 +229,61 @@ class Context:
         ms = np.zeros(NTILECFG)
         self._check(self._L.hc_get_tiles(self._h, C.byref(cfg), _p(ms)), "hc_get_tiles")
         return cfg.value, ms
+
+    # ---- derivatives (single case, FP64) ----
+    def _brows(self):
+        return 2 * self.nj + 2 * self.ni
+
+    def tangent_residual(self, qdot, bref_dot=None, minf_dot=0.0):
+        """(R(q) [ni][nj][nv+1], Rdot [ni][nj][nv]) at the current state."""
+        qdot = np.ascontiguousarray(qdot, dtype=np.float64)
+        assert qdot.size == self.ni * self.nj * self.nv
+        res = np.zeros((self.ni, self.nj, self.nv + 1))
+        resd = np.zeros((self.ni, self.nj, self.nv))
+        bd = None
+        if bref_dot is not None:
+            bref_dot = np.ascontiguousarray(bref_dot, dtype=np.float64)
+            assert bref_dot.size == self._brows() * self.nv
+            bd = _p(bref_dot)
+        self._check(self._L.hc_tangent_residual(self._h, _p(qdot), bd, float(minf_dot), _p(res), _p(resd)),
+                    "hc_tangent_residual")
+        return res, resd
+
+    def adjoint_residual(self, ybar):
+        """(qbar [ni][nj][nv], bref_bar [rows][nv], minf_bar) at the current state."""
+        ybar = np.ascontiguousarray(ybar, dtype=np.float64)
+        assert ybar.size == self.ni * self.nj * self.nv
+        qbar = np.zeros((self.ni, self.nj, self.nv))
+        bb = np.zeros((self._brows(), self.nv))
+        mb = C.c_double()
+        self._check(self._L.hc_adjoint_residual(self._h, _p(ybar), _p(qbar), _p(bb), C.byref(mb)),
+                    "hc_adjoint_residual")
+        return qbar, bb, mb.value
+
+    def adjoint_forward(self, nsteps, checkpoint_every):
+        self._check(self._L.hc_adjoint_forward(self._h, int(nsteps), int(checkpoint_every)), "hc_adjoint_forward")
+
+    def adjoint_backward(self, qbar_final):
+        qbar_final = np.ascontiguousarray(qbar_final, dtype=np.float64)
+        assert qbar_final.size == self.ni * self.nj * self.nv
+        q0b = np.zeros((self.ni, self.nj, self.nv))
+        bb = np.zeros((self._brows(), self.nv))
+        mb = C.c_double()
+        self._check(self._L.hc_adjoint_backward(self._h, _p(qbar_final), _p(q0b), _p(bb), C.byref(mb)),
+                    "hc_adjoint_backward")
+        return q0b, bb, mb.value
+
+    def tangent_steps(self, nsteps, qdot0, bref_dot=None, minf_dot=0.0):
+        qdot0 = np.ascontiguousarray(qdot0, dtype=np.float64)
+        assert qdot0.size == self.ni * self.nj * self.nv
+        out = np.zeros((self.ni, self.nj, self.nv))
+        bd = None
+        if bref_dot is not None:
+            bref_dot = np.ascontiguousarray(bref_dot, dtype=np.float64)
+            bd = _p(bref_dot)
+        self._check(self._L.hc_tangent_steps(self._h, int(nsteps), _p(qdot0), bd, float(minf_dot), _p(out)),
+                    "hc_tangent_steps")
+        return out
 
     def stream_triad(self, n, reps):
         g = C.c_double()

@@ -10,7 +10,7 @@ import math
 
 import numpy as np
 
-from . import (BASIS_CONSERVED, BASIS_PRIMITIVE, BC_PERIODIC, BC_RIEMANN, BC_WALL, DT_GLOBAL, DT_LOCAL,
+from . import (BASIS_CONSERVED, BASIS_PRIMITIVE, BC_PERIODIC, BC_RIEMANN, BC_WALL, DT_FIXED, DT_GLOBAL, DT_LOCAL,
                FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MINMOD2,
                LIM_SMOOTHVANALBADA, LIM_VANALBADA2, LIM_VANLEER2, SCALE_ACOUSTIC, SCALE_CONVECTIVE, CaseDesc,
                Species)
@@ -302,4 +302,77 @@ def config_cylinder(nth, nr, mach, alpha_deg=0.0, flux=FLUX_ROE, nsteps=1000):
                    "rk4", 1.0, DT_LOCAL, air_species(mach))
     return dict(name=f"euler-cylinder-{nth}x{nr}-M{mach:.3f}-a{alpha_deg:+.2f}", nodes=nodes, q=q, case=cs,
                 bref=bref, nsteps=nsteps)
+
+
+# ---------------- adjoint case: cylinder, AUSM+up, smooth van Albada ----------------
+
+def smooth_perturbation(nr, nth, seed=4242, amp=1e-3, modes=4):
+    """Smooth field on the O-grid, per conserved variable k:
+    amp * sum_{m,n=1..modes} a_kmn sin(pi n (i+1/2)/nr) cos(m theta_j + phi_kmn), theta_j = 2 pi (j+1/2)/nth,
+    a ~ U[-1,1]/16 and phi ~ U[0, 2 pi) drawn from MT19937-64(seed) in the order k, m, n, (a, phi)."""
+    rng = MT19937_64(seed)
+    u = lambda: (rng() >> 11) * 2.0 ** -53
+    i = (np.arange(nr) + 0.5)[:, None]
+    th = (2.0 * math.pi * (np.arange(nth) + 0.5) / nth)[None, :]
+    out = np.zeros((nr, nth, 4))
+    for k in range(4):
+        for m in range(1, modes + 1):
+            for n in range(1, modes + 1):
+                a = (2.0 * u() - 1.0) / 16.0
+                phi = 2.0 * math.pi * u()
+                out[..., k] += a * np.sin(math.pi * n * i / nr) * np.cos(m * th + phi)
+    return amp * out
+
+
+def freestream_cons(mach, alpha_deg=0.0):
+    a = math.radians(alpha_deg)
+    return prim_to_cons(np.array([mach * math.cos(a), mach * math.sin(a), 1.0, 1.0 / GAMMA]))
+
+
+def config_adjoint(nth=1024, nr=512, mach=0.4, nsteps=400, checkpoint_every=20, dt=None):
+    """Cylinder O-grid (the C2 rule at n_theta x n_r), M = 0.4, AUSM+up, smooth van Albada,
+    SSP-RK3 with a fixed time step, conserved basis; initial state = free stream + the seeded
+    smooth perturbation.  dt = None leaves the step to the caller (CFL 0.8 of the initial state)."""
+    nodes = mesh_ogrid(nr, nth)
+    qinf = freestream_cons(mach)
+    q = qinf[None, None, :] + smooth_perturbation(nr, nth)
+    bref = np.broadcast_to(qinf, (bref_size(nr, nth), 4)).copy()
+    cs = make_case(LAW_EULER, BASIS_CONSERVED, FLUX_AUSMPUP, LIM_SMOOTHVANALBADA,
+                   (BC_WALL, BC_RIEMANN, BC_PERIODIC, BC_PERIODIC), "ssp33", 1.0 if dt is None else dt, DT_FIXED,
+                   air_species(mach))
+    return dict(name=f"euler-cylinder-{nth}x{nr}-ausmpup-smoothvanalbada-ssprk3-adjoint", nodes=nodes, q=q, case=cs,
+                bref=bref, nsteps=nsteps, checkpoint_every=checkpoint_every, mach=mach)
+
+
+def wall_faces(nodes):
+    """Area and unit normal (into the fluid) of the wall faces on node line 0
+    (geometry/surface.ipp:12-29 with the boundary orientation of boundaryFluxLoop.h:162-175)."""
+    p0, p1 = nodes[0, :-1], nodes[0, 1:]
+    t0 = p1 - p0
+    l = np.sqrt(t0[:, 0] * t0[:, 0] + t0[:, 1] * t0[:, 1])
+    s1 = (p0 - p1) * (1.0 / l)[:, None]
+    return l, -s1[:, 1], s1[:, 0]
+
+
+def wall_force_x(nodes, q, gamma=GAMMA):
+    """J = -sum over wall faces of p_w A n_x (n into the fluid), p_w the pressure of the wall cell
+    (conserved basis, inviscidWall.ipp: the wall state is the interior state)."""
+    area, nx, _ = wall_faces(nodes)
+    qw = q[0]
+    p = (gamma - 1.0) * (qw[:, 3] - 0.5 * (qw[:, 0] ** 2 + qw[:, 1] ** 2) / qw[:, 2])
+    return -np.sum(p * area * nx)
+
+
+def wall_force_x_grad(nodes, q, gamma=GAMMA):
+    """dJ/dq of wall_force_x (nonzero on the wall cells only)."""
+    area, nx, _ = wall_faces(nodes)
+    qw = q[0]
+    u, v = qw[:, 0] / qw[:, 2], qw[:, 1] / qw[:, 2]
+    g = np.zeros_like(q)
+    w = -(area * nx) * (gamma - 1.0)
+    g[0, :, 0] = w * (-u)
+    g[0, :, 1] = w * (-v)
+    g[0, :, 2] = w * (0.5 * (u * u + v * v))
+    g[0, :, 3] = w
+    return g
 

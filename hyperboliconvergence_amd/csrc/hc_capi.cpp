@@ -3,10 +3,7 @@
 // norms.  A context holds one case (hc_set_mesh / hc_set_case) or a batch of cases
 // (hc_batch_begin / hc_batch_add_case / hc_batch_end) that share the flux, limiter, basis
 // and Runge-Kutta table; every stage launch covers the tiles of all its cases.
-#include "hcamd.h"
-
-#include "hc_geom.hpp"
-#include "hc_launch.hpp"
+#include "hc_context.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -123,79 +120,9 @@ template <class T> void free_dev(T*& p)
 }   // namespace
 
 
-namespace {
+namespace hc_detail {
 
-// host-side record of one case of a context
-struct CaseHost {
-    int ni = 0, nj = 0;
-    hc_case_desc cs{};
-    GeomOutput g;                  // host geometry until it is uploaded
-    long long N = 0, nfi = 0, nfj = 0;
-    long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
-    int brows = 0;                 // boundary reference rows 2 nj + 2 ni
-    bool have_state = false;
-    int tiles_j[NTILECFG] = {}, tile0[NTILECFG] = {}, ntiles[NTILECFG] = {};
-};
-
-}   // namespace
-
-struct hc_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // single-case staging (hc_set_mesh)
-    int ni = 0, nj = 0;
-    std::vector<double> nodes;
-    bool have_mesh = false;
-    // cases
-    bool building = false;         // between hc_batch_begin and hc_batch_end
-    std::vector<CaseHost> cases;
-    bool have_case = false;        // device layout built
-    hc_case_desc cs{};             // parameters shared by the cases (case 0's descriptor)
-    int nv = 0;
-    int precision = HC_PRECISION_FP64;
-    std::vector<RkPlan> plans;
-    int nslots = 0;
-    long long Ntot = 0, nfi_tot = 0, nfj_tot = 0, brows_tot = 0;
-    // device buffers
-    double *cvol = nullptr, *ccx = nullptr, *ccy = nullptr, *fi = nullptr, *fj = nullptr;
-    float *fi32 = nullptr, *fj32 = nullptr;
-    double* q[3] = {nullptr, nullptr, nullptr};
-    int cur = 0;
-    double *acc = nullptr, *tmp = nullptr, *lam0 = nullptr, *bref = nullptr, *res = nullptr, *part = nullptr,
-           *nout = nullptr;
-    unsigned long long* maxslot = nullptr;   // one per case
-    double* dtslot = nullptr;                // two per case
-    unsigned int* status = nullptr;
-    CaseRec<double>* rec64[NTILECFG] = {};
-    CaseRec<float>* rec32[NTILECFG] = {};
-    int* tile_case[NTILECFG] = {};
-    int ntiles[NTILECFG] = {};
-    UpdChunk* chunks = nullptr;
-    int nchunks = 0;
-    // tile shape: fixed index, or -1 to time every shape on the first steps and keep the fastest
-    int tile_mode = -1;
-    int tile_cfg = 0;
-    bool tuned = false;
-    double tune_ms[NTILECFG] = {};
-    // graphs
-    int graph_steps = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    int graph_cur = -1, graph_cfg = -1;
-    // profiling
-    int profiling = 0;
-    double prof[4] = {0, 0, 0, 0};
-    std::vector<hipEvent_t> ev_pool;
-    struct Pending {
-        hipEvent_t e0, e1;
-        double n_stage, n_update;   // launches the interval covers
-        int kind;                   // 0 stage kernel, 1 update kernel, 2 whole hc_step call
-    };
-    std::vector<Pending> ev_pending;
-};
-
-namespace {
+using namespace hc;
 
 int fail(hc_context* c, int code, const std::string& msg)
 {
@@ -208,11 +135,6 @@ int hipfail(hc_context* c, hipError_t e, const char* where)
     return fail(c, HC_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
 }
 
-#define HC_HIP(call)                                           \
-    do {                                                       \
-        hipError_t e_ = (call);                                \
-        if (e_ != hipSuccess) return hipfail(ctx, e_, #call);  \
-    } while (0)
 
 void drop_graph(hc_context* ctx)
 {
@@ -252,6 +174,7 @@ void release_buffers(hc_context* ctx)
     ctx->nchunks = 0;
     ctx->have_case = false;
     ctx->tuned = false;
+    release_adjoint(ctx);
 }
 
 bool valid_ids(const hc_case_desc& d)
@@ -797,7 +720,11 @@ int all_states(hc_context* ctx)
     return HC_OK;
 }
 
-}   // namespace
+template hc::LaunchArgs<double> launch_args<double>(hc_context* ctx);
+
+}   // namespace hc_detail
+
+using namespace hc_detail;
 
 extern "C" {
 

@@ -1,0 +1,116 @@
+// hc_context.hpp -- the context behind the C-ABI (include/hcamd.h) and the internal helpers
+// shared by the translation units of the library (hc_capi.cpp, hc_adjoint.cpp).
+#pragma once
+
+#include "hcamd.h"
+
+#include "hc_geom.hpp"
+#include "hc_launch.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+using hc::CaseRec;
+using hc::GeomOutput;
+using hc::NTILECFG;
+using hc::RkPlan;
+using hc::UpdChunk;
+
+struct AdjState;   // buffers of the tangent / adjoint entry points (hc_adjoint.cpp)
+
+namespace hc_detail {
+
+// host-side record of one case of a context
+struct CaseHost {
+    int ni = 0, nj = 0;
+    hc_case_desc cs{};
+    GeomOutput g;                  // host geometry until it is uploaded
+    long long N = 0, nfi = 0, nfj = 0;
+    long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
+    int brows = 0;                 // boundary reference rows 2 nj + 2 ni
+    bool have_state = false;
+    int tiles_j[NTILECFG] = {}, tile0[NTILECFG] = {}, ntiles[NTILECFG] = {};
+};
+
+}   // namespace hc_detail
+
+struct hc_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // single-case staging (hc_set_mesh)
+    int ni = 0, nj = 0;
+    std::vector<double> nodes;
+    bool have_mesh = false;
+    // cases
+    bool building = false;         // between hc_batch_begin and hc_batch_end
+    std::vector<hc_detail::CaseHost> cases;
+    bool have_case = false;        // device layout built
+    hc_case_desc cs{};             // parameters shared by the cases (case 0's descriptor)
+    int nv = 0;
+    int precision = HC_PRECISION_FP64;
+    std::vector<RkPlan> plans;
+    int nslots = 0;
+    long long Ntot = 0, nfi_tot = 0, nfj_tot = 0, brows_tot = 0;
+    // device buffers
+    double *cvol = nullptr, *ccx = nullptr, *ccy = nullptr, *fi = nullptr, *fj = nullptr;
+    float *fi32 = nullptr, *fj32 = nullptr;
+    double* q[3] = {nullptr, nullptr, nullptr};
+    int cur = 0;
+    double *acc = nullptr, *tmp = nullptr, *lam0 = nullptr, *bref = nullptr, *res = nullptr, *part = nullptr,
+           *nout = nullptr;
+    unsigned long long* maxslot = nullptr;   // one per case
+    double* dtslot = nullptr;                // two per case
+    unsigned int* status = nullptr;
+    CaseRec<double>* rec64[NTILECFG] = {};
+    CaseRec<float>* rec32[NTILECFG] = {};
+    int* tile_case[NTILECFG] = {};
+    int ntiles[NTILECFG] = {};
+    UpdChunk* chunks = nullptr;
+    int nchunks = 0;
+    // tile shape: fixed index, or -1 to time every shape on the first steps and keep the fastest
+    int tile_mode = -1;
+    int tile_cfg = 0;
+    bool tuned = false;
+    double tune_ms[NTILECFG] = {};
+    // graphs
+    int graph_steps = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_cur = -1, graph_cfg = -1;
+    // profiling
+    int profiling = 0;
+    double prof[4] = {0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    struct Pending {
+        hipEvent_t e0, e1;
+        double n_stage, n_update;   // launches the interval covers
+        int kind;                   // 0 stage kernel, 1 update kernel, 2 whole hc_step call
+    };
+    std::vector<Pending> ev_pending;
+    // tangent / adjoint buffers, created on first use
+    AdjState* adj = nullptr;
+};
+
+
+namespace hc_detail {
+
+int fail(hc_context* c, int code, const std::string& msg);
+int hipfail(hc_context* c, hipError_t e, const char* where);
+int ready(hc_context* ctx);
+int case_ready(hc_context* ctx, int k);
+int check_status(hc_context* ctx);
+int enqueue_step(hc_context* ctx);
+hipError_t launch_stage(hc_context* ctx, const hc::LaunchArgs<double>& L);
+template <class RF> hc::LaunchArgs<RF> launch_args(hc_context* ctx);
+void release_adjoint(hc_context* ctx);
+
+}   // namespace hc_detail
+
+#define HC_HIP(call)                                                      \
+    do {                                                                  \
+        hipError_t e_ = (call);                                           \
+        if (e_ != hipSuccess) return hc_detail::hipfail(ctx, e_, #call);  \
+    } while (0)

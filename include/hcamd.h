@@ -202,6 +202,33 @@ int hc_batch_get_time(hc_context* ctx, int case_id, double* t);
 int hc_set_tiles(hc_context* ctx, int config);
 int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step);
 
+/*
+ * Derivatives (single-case context, FP64; Euler with Rusanov, Roe, SLAU or AUSM+up in the
+ * conserved basis, AUSM+up also in the primitive basis).  No counterpart in the reference.
+ * Parameters: the state q, minf of the species (AUSM+up / SLAU scaling) and the boundary
+ * reference states bref ([rows][nvar] as hc_set_boundary_refs).
+ *
+ * hc_tangent_residual: R(q) of the current state ([N][nvar+1], as hc_residual) and
+ *   Rdot = dR/dq qdot + dR/dbref bref_dot + dR/dminf minf_dot ([N][nvar]); bref_dot may be null.
+ * hc_adjoint_residual: qbar = (dR/dq)^T ybar ([N][nvar]), bref_bar = (dR/dbref)^T ybar and
+ *   minf_bar = (dR/dminf)^T ybar, for ybar [N][nvar].
+ *
+ * Trajectories (conserved basis, HC_DT_FIXED): hc_adjoint_forward advances the current state
+ * nsteps steps and keeps a checkpoint every checkpoint_every steps; hc_adjoint_backward then
+ * maps the adjoint of the final state to the adjoint of the initial state and the adjoints of
+ * bref and minf accumulated over all stages, recomputing each segment's states from its
+ * checkpoint.  hc_tangent_steps advances the state nsteps steps together with its tangent
+ * (initial qdot0, parameter tangents bref_dot, minf_dot) and returns the final tangent.
+ */
+int hc_tangent_residual(hc_context* ctx, const double* qdot, const double* bref_dot, double minf_dot, double* res,
+                        double* res_dot);
+int hc_adjoint_residual(hc_context* ctx, const double* ybar, double* qbar, double* bref_bar, double* minf_bar);
+int hc_adjoint_forward(hc_context* ctx, int nsteps, int checkpoint_every);
+int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_initial, double* bref_bar,
+                        double* minf_bar);
+int hc_tangent_steps(hc_context* ctx, int nsteps, const double* qdot0, const double* bref_dot, double minf_dot,
+                     double* qdot_final);
+
 /* Library version string. */
 const char* hc_version(void);
 

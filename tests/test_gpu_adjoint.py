@@ -1,0 +1,131 @@
+"""GPU: tangent and adjoint kernels through the C-ABI.
+
+* residual level: the device kernels against their host execution (same code) and the
+  dot-product identity between the device tangent and adjoint;
+* trajectory level (fixed dt, SSP-RK3, AUSM+up with the smooth van Albada limiter on the
+  cylinder O-grid): the adjoint of K steps against the tangent of K steps (dot-product
+  identity), against central finite differences of the oracle trajectory (h = 1e-6), and for
+  several checkpoint intervals (bit-identical gradients).
+"""
+import numpy as np
+import pytest
+
+import pyemu
+import pyoracle as po
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg(nth=32, nr=16, nsteps=12):
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_adjoint(nth, nr, nsteps=nsteps)
+    res = po.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"])
+    vol = po.dual(cfg["nodes"])[..., 0]
+    cfg["case"].cfl = 0.8 / np.max(res[..., 4] / vol)   # fixed step: CFL 0.8 of the initial state
+    return cfg
+
+
+def _ctx(cfg):
+    import hyperboliconvergence_amd as hca
+    ctx = hca.Context(0)
+    ctx.set_mesh(cfg["nodes"])
+    ctx.set_case(cfg["case"])
+    ctx.set_tiles(0)
+    ctx.set_boundary_refs(cfg["bref"])
+    ctx.set_state(cfg["q"])
+    return ctx
+
+
+def test_tangent_and_adjoint_residual_match_host_execution():
+    cfg = _cfg()
+    rng = np.random.default_rng(1)
+    qd = rng.standard_normal(cfg["q"].shape)
+    bd = rng.standard_normal(cfg["bref"].shape)
+    yb = rng.standard_normal(cfg["q"].shape)
+    with _ctx(cfg) as ctx:
+        res, rd = ctx.tangent_residual(qd, bd, 0.3)
+        qb, bb, mb = ctx.adjoint_residual(yb)
+    res_e, rd_e = pyemu.tangent_residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], qd, bd, 0.3)
+    qb_e, bb_e, mb_e = pyemu.adjoint_residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], yb)
+    assert rel_err(res, res_e) <= 1e-12 and rel_err(rd, rd_e) <= 1e-12
+    assert rel_err(qb, qb_e) <= 1e-12 and rel_err(bb, bb_e) <= 1e-12 and abs(mb - mb_e) <= 1e-12 * abs(mb_e)
+    lhs = np.sum(yb * rd)
+    rhs = np.sum(qb * qd) + np.sum(bb * bd) + mb * 0.3
+    assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(yb * rd))
+
+
+def _adjoint_grad(cfg, K, C):
+    from hyperboliconvergence_amd import cases
+    with _ctx(cfg) as ctx:
+        ctx.adjoint_forward(K, C)
+        qK = ctx.get_state()
+        g = cases.wall_force_x_grad(cfg["nodes"], qK)
+        q0b, bb, mb = ctx.adjoint_backward(g)
+    return qK, g, q0b, bb, mb
+
+
+def test_trajectory_adjoint_is_transpose_of_trajectory_tangent():
+    cfg = _cfg()
+    K = cfg["nsteps"]
+    qK, g, q0b, bb, mb = _adjoint_grad(cfg, K, 4)
+    rng = np.random.default_rng(2)
+    qd = rng.standard_normal(cfg["q"].shape) * 1e-3
+    bd = rng.standard_normal(cfg["bref"].shape) * 1e-3
+    md = 0.7
+    with _ctx(cfg) as ctx:
+        qdK = ctx.tangent_steps(K, qd, bd, md)
+        assert np.array_equal(ctx.get_state(), qK)
+    lhs = np.sum(g * qdK)
+    rhs = np.sum(q0b * qd) + np.sum(bb * bd) + mb * md
+    print("tangent %.15e adjoint %.15e" % (lhs, rhs))
+    assert abs(lhs - rhs) <= 1e-10 * (np.sum(np.abs(q0b * qd)) + np.sum(np.abs(bb * bd)) + abs(mb * md))
+
+
+def test_checkpoint_interval_does_not_change_the_gradient():
+    cfg = _cfg(nsteps=10)
+    ref = _adjoint_grad(cfg, 10, 10)
+    for C in (1, 3):
+        out = _adjoint_grad(cfg, 10, C)
+        assert np.array_equal(out[2], ref[2]) and np.array_equal(out[3], ref[3]) and out[4] == ref[4]
+
+
+def test_trajectory_gradient_matches_oracle_finite_differences():
+    """Wall force J(q_K) for the state and minf; a weighted sum of q_K (which sees the far field
+    within K steps) for the boundary references.  The state direction is a smooth field of
+    amplitude 1e-3 (the perturbation family of the adjoint case); at h = 1e-5 the central
+    difference is within 2e-7 of the derivative (round-off of J's cancellations grows below)."""
+    from hyperboliconvergence_amd import cases
+    cfg = _cfg()
+    K = cfg["nsteps"]
+    _, _, q0b, bb, mb = _adjoint_grad(cfg, K, 4)
+
+    def traj(q0, bref, minf):
+        cs = type(cfg["case"]).from_buffer_copy(cfg["case"])
+        cs.sp.minf = minf
+        qK, _, _ = po.integrate(cs, cfg["nodes"], q0, K, bref)
+        return qK
+
+    def J(q0, bref, minf):
+        return cases.wall_force_x(cfg["nodes"], traj(q0, bref, minf))
+
+    d = cases.smooth_perturbation(16, 32, seed=7, amp=1e-3)
+    h = 1e-5
+    fd = (J(cfg["q"] + h * d, cfg["bref"], 0.4) - J(cfg["q"] - h * d, cfg["bref"], 0.4)) / (2 * h)
+    ad = np.sum(q0b * d)
+    print("dJ/dq0 . d: adjoint %.12e fd %.12e" % (ad, fd))
+    assert abs(ad - fd) <= 1e-6 * abs(fd)
+    h = 1e-6
+    fd_m = (J(cfg["q"], cfg["bref"], 0.4 + h) - J(cfg["q"], cfg["bref"], 0.4 - h)) / (2 * h)
+    print("dJ/dminf: adjoint %.12e fd %.12e" % (mb, fd_m))
+    assert abs(mb - fd_m) <= 1e-6 * abs(fd_m)
+    # weighted sum of the final state: the boundary references
+    w = np.random.default_rng(3).standard_normal(cfg["q"].shape)
+    with _ctx(cfg) as ctx:
+        ctx.adjoint_forward(K, 4)
+        _, bbw, _ = ctx.adjoint_backward(w)
+    e = np.random.default_rng(4).standard_normal(cfg["bref"].shape) * 1e-3
+    h = 1e-5
+    fd_b = (np.sum(w * traj(cfg["q"], cfg["bref"] + h * e, 0.4)) - np.sum(w * traj(cfg["q"], cfg["bref"] - h * e, 0.4))) / (2 * h)
+    print("d(w.qK)/dbref . e: adjoint %.12e fd %.12e" % (np.sum(bbw * e), fd_b))
+    assert abs(np.sum(bbw * e) - fd_b) <= 1e-6 * abs(fd_b)
