@@ -120,6 +120,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     ctx.set_profiling(0)
     tiles = ctx.tiles()
     q = ctx.get_state()
+    tsim = ctx.time()
     st = ctx.status()
     ctx.status()   # clear
     # per-launch breakdown (events around every launch, graphs off)
@@ -129,7 +130,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     per_launch = ctx.profile()
     ctx.set_profiling(0)
     ctx.close()
-    return t1 - t0, q, st, (region, per_launch, tiles), warmup
+    return t1 - t0, q, st, (region, per_launch, tiles, tsim), warmup
 
 
 def cpu_baseline(cfg, budget_s=12.0):
@@ -303,12 +304,101 @@ def main_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def main_adjoint(a):
+    """Adjoint workload: 1024 x 512 cylinder O-grid, AUSM+up, smooth van Albada, SSP-RK3 with a
+    fixed step (CFL 0.8 of the initial state), K steps with a checkpoint every C; the gradient of
+    the wall force in x with respect to the initial state and the free-stream Mach number of the
+    species, checked against central differences of GPU FP64 trajectories and the tangent."""
+    import hyperboliconvergence_amd as hca
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_adjoint()
+    K = a.steps if a.steps is not None else cfg["nsteps"]
+    C = cfg["checkpoint_every"]
+    nodes, q0, bref = cfg["nodes"], cfg["q"], cfg["bref"]
+    ni, nj = q0.shape[:2]
+    N = ni * nj
+    ns = cfg["case"].nstages
+    ctx = hca.Context(0)
+    ctx.set_mesh(nodes)
+    ctx.set_case(cfg["case"])
+    ctx.set_boundary_refs(bref)
+    ctx.set_state(q0)
+    lam = ctx.residual()[..., 4]
+    dt = 0.8 / float(np.max(lam / cases.cell_volumes(nodes)))
+    cfg["case"].cfl = dt
+    ctx.set_case(cfg["case"])
+    ctx.set_tiles(0)
+
+    def run(qs, minf=None):
+        if minf is not None:
+            cs = type(cfg["case"]).from_buffer_copy(cfg["case"])
+            cs.sp.minf = minf
+            ctx.set_case(cs)
+            ctx.set_tiles(0)
+        ctx.set_boundary_refs(bref)
+        ctx.set_state(qs)
+        ctx.adjoint_forward(K, C)
+        out = ctx.get_state()
+        if minf is not None:
+            ctx.set_case(cfg["case"])
+            ctx.set_tiles(0)
+        return out
+
+    run(q0)   # warm-up
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    qK = run(q0)
+    ctx.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g = cases.wall_force_x_grad(nodes, qK)
+    J = cases.wall_force_x(nodes, qK)
+    t0 = time.perf_counter()
+    q0b, bb, mb = ctx.adjoint_backward(g)
+    t_bwd = time.perf_counter() - t0
+    print(f"# adjoint: forward {K} steps {t_fwd:.3f} s, backward {t_bwd:.3f} s, J = {J:.12e}", file=sys.stderr,
+          flush=True)
+    d = cases.smooth_perturbation(ni, nj, seed=7, amp=1e-3)
+    ad = float(np.sum(q0b * d))
+    fd = {}
+    for h in (1e-6, 1e-5):
+        jp = cases.wall_force_x(nodes, run(q0 + h * d))
+        jm = cases.wall_force_x(nodes, run(q0 - h * d))
+        fd[h] = (jp - jm) / (2 * h)
+    h = 1e-6
+    fdm = (cases.wall_force_x(nodes, run(q0, cfg["mach"] + h)) - cases.wall_force_x(nodes, run(q0, cfg["mach"] - h))) / (2 * h)
+    ctx.set_boundary_refs(bref)
+    ctx.set_state(q0)
+    t0 = time.perf_counter()
+    qdK = ctx.tangent_steps(K, d)
+    t_tan = time.perf_counter() - t0
+    tan = float(np.sum(g * qdK))
+    st = ctx.status()
+    ctx.close()
+    rel = lambda x, y: abs(x - y) / abs(y)
+    print(f"# dJ/dq0.d adjoint {ad:.12e} tangent {tan:.12e} fd(1e-6) {fd[1e-6]:.12e} fd(1e-5) {fd[1e-5]:.12e}; "
+          f"dJ/dminf adjoint {mb:.12e} fd {fdm:.12e}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "cell-stage updates/s + %HBM roofline; mixed-FP32 vs FP64 at ≤1e-6 rel err",
+        "value": N * ns * K / t_fwd, "unit": "cell-stage updates/s", "n_gpus": 1, "steps": K, "warmup": K,
+        "ms_per_step": t_fwd / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": cfg["name"], "cells": N, "rk_stages": ns, "checkpoint_every": C, "dt": dt},
+        "adjoint": {"forward_s": t_fwd, "backward_s": t_bwd, "tangent_s": t_tan, "backward_over_forward": t_bwd / t_fwd,
+                    "J": J, "dJdq0_dot_d": ad, "tangent_dot": tan, "fd_h1e-6": fd[1e-6], "fd_h1e-5": fd[1e-5],
+                    "rel_err_vs_fd_h1e-6": rel(ad, fd[1e-6]), "rel_err_vs_fd_h1e-5": rel(ad, fd[1e-5]),
+                    "rel_err_vs_tangent": rel(ad, tan), "dJdminf": mb, "dJdminf_fd_h1e-6": fdm,
+                    "dJdminf_rel_err": rel(mb, fdm), "bref_bar_norm": float(np.linalg.norm(bb))},
+        "status": int(st),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
@@ -325,6 +415,8 @@ def main():
 
     if a.workload == "c4":
         return main_batch(a)
+    if a.workload == "c5":
+        return main_adjoint(a)
     cfg = make_cfg(a.workload, a.steps)
     steps = a.steps if a.steps is not None else cfg["bench_steps"]
     cs = cfg["case"]
@@ -337,14 +429,14 @@ def main():
 
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
-        el, q, st, (region, per_launch, tiles), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
+        el, q, st, (region, per_launch, tiles, tsim), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
         # stage-kernel time: the events' interval over the timed steps per stage launch (it also
         # holds the global-dt update kernels and the gaps between launches: an upper bound)
         kern_ms = region[0] / max(region[1], 1.0)
         res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms,
                          launch_ms=per_launch[0] / max(per_launch[1], 1.0),
                          upd_ms=per_launch[2] / max(per_launch[3], 1.0), rate=N * ns * steps / el,
-                         tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]])
+                         tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]], tsim=tsim)
         print(f"# {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, "
               f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}, "
               f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})",
@@ -365,6 +457,22 @@ def main():
     head = 1 if (1 in res and mixed_err is not None and mixed_err <= 1e-6) else 0
     h = res[head]
 
+    # isentropic vortex: L2 density error against the analytic solution at the physical time
+    # (cell volumes are area/4 in the reference, so the physical time is 4 t)
+    vortex = None
+    if a.workload == "c3":
+        from hyperboliconvergence_amd import cases
+        cc = cases.cell_centres(cfg["nodes"])
+        vortex = {}
+        for prec, r in res.items():
+            exact = cases.isentropic_vortex(cc[..., 0], cc[..., 1], t=4.0 * r["tsim"])
+            vortex["fp64" if prec == 0 else "mixed"] = {
+                "t_physical": 4.0 * r["tsim"],
+                "l2_density_error_vs_analytic": float(np.sqrt(np.mean((r["q"][..., 2] - exact[..., 2]) ** 2)))}
+        if 0 in res and 1 in res:
+            vortex["l2_density_mixed_vs_fp64"] = float(np.sqrt(np.mean((res[1]["q"][..., 2] - res[0]["q"][..., 2]) ** 2)))
+        print(f"# vortex: {vortex}", file=sys.stderr, flush=True)
+
     # the same workload with the face-frame Roe variant (stable on the O-grid)
     ff = None
     if a.workload == "c2" and a.precision == "both" and not a.no_extras:
@@ -375,7 +483,7 @@ def main():
         ff = {}
         ff_steps = cfg["nsteps"]
         for prec in (0, 1):
-            el, q, st, (region, _, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
+            el, q, st, (region, _, _, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
             ff[prec] = dict(rate=N * ns * ff_steps / el, q=q, status=st, kernel_us=region[0] / max(region[1], 1.0) * 1e3)
         ff_err = rel_err_conserved(ff[1]["q"], ff[0]["q"], cs.law, cs.basis)
         print(f"# face-frame Roe: fp64 {ff[0]['rate']:.4e}, mixed {ff[1]['rate']:.4e} cell-stage/s, "
@@ -429,6 +537,8 @@ def main():
         "status": int(h["status"]),
         "face_frame_roe": ff,
     }
+    if vortex is not None:
+        out["isentropic_vortex"] = vortex
     print(json.dumps(out), flush=True)
 
 

@@ -344,6 +344,16 @@ def config_adjoint(nth=1024, nr=512, mach=0.4, nsteps=400, checkpoint_every=20, 
                 bref=bref, nsteps=nsteps, checkpoint_every=checkpoint_every, mach=mach)
 
 
+def cell_volumes(nodes):
+    """Cell volumes as the reference defines them: |J(0,0)| of the bilinear map = area / 4
+    (geometry/volume.ipp); shoelace area of the cell quadrilateral."""
+    p0, p1, p2, p3 = nodes[:-1, :-1], nodes[1:, :-1], nodes[1:, 1:], nodes[:-1, 1:]
+    x = [p0[..., 0], p1[..., 0], p2[..., 0], p3[..., 0]]
+    y = [p0[..., 1], p1[..., 1], p2[..., 1], p3[..., 1]]
+    area = 0.5 * np.abs(sum(x[k] * y[(k + 1) % 4] - x[(k + 1) % 4] * y[k] for k in range(4)))
+    return area / 4.0
+
+
 def wall_faces(nodes):
     """Area and unit normal (into the fluid) of the wall faces on node line 0
     (geometry/surface.ipp:12-29 with the boundary orientation of boundaryFluxLoop.h:162-175)."""
