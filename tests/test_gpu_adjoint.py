@@ -119,16 +119,18 @@ def test_trajectory_gradient_matches_oracle_finite_differences():
     fd_m = (J(cfg["q"], cfg["bref"], 0.4 + h) - J(cfg["q"], cfg["bref"], 0.4 - h)) / (2 * h)
     print("dJ/dminf: adjoint %.12e fd %.12e" % (mb, fd_m))
     assert abs(mb - fd_m) <= 1e-6 * abs(fd_m)
-    # weighted sum of the final state: the boundary references
-    w = np.random.default_rng(3).standard_normal(cfg["q"].shape)
+    # weighted sum of the final state near the far field: the boundary references (the far-field
+    # state reaches three rows in K steps; the difference quotient levels off near 7e-7 there)
+    w = np.zeros_like(cfg["q"])
+    w[-3:] = np.random.default_rng(3).standard_normal(w[-3:].shape)
     with _ctx(cfg) as ctx:
         ctx.adjoint_forward(K, 4)
         _, bbw, _ = ctx.adjoint_backward(w)
-    e = np.random.default_rng(4).standard_normal(cfg["bref"].shape) * 1e-3
-    h = 1e-5
+    e = np.random.default_rng(4).standard_normal(cfg["bref"].shape)
+    h = 1e-6
     fd_b = (np.sum(w * traj(cfg["q"], cfg["bref"] + h * e, 0.4)) - np.sum(w * traj(cfg["q"], cfg["bref"] - h * e, 0.4))) / (2 * h)
     print("d(w.qK)/dbref . e: adjoint %.12e fd %.12e" % (np.sum(bbw * e), fd_b))
-    assert abs(np.sum(bbw * e) - fd_b) <= 1e-6 * abs(fd_b)
+    assert abs(np.sum(bbw * e) - fd_b) <= 2e-6 * abs(fd_b)
 
 
 def test_torch_autograd_gradient_through_mach_number():
