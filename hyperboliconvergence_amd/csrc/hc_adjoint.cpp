@@ -176,6 +176,8 @@ AdjArgs base_adj(hc_context* ctx)
     a.m.ccy = ctx->ccy;
     a.m.fi = ctx->fi;
     a.m.fj = ctx->fj;
+    a.m.fi64 = ctx->fi;
+    a.m.fj64 = ctx->fj;
     const hc_species& sp = c.cs.sp;
     a.sp = Species<double>{sp.gamma, sp.minf, sp.lref, sp.nu, sp.pr, sp.dt, sp.R, sp.gamma1};
     a.vscale = c.cs.vscale;

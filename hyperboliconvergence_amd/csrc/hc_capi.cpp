@@ -253,6 +253,8 @@ template <class RF> void fill_rec(const hc_context* ctx, const CaseHost& c, int 
         r.m.fi = ctx->fi32 + NSLOT * c.fi_off;
         r.m.fj = ctx->fj32 + NSLOT * c.fj_off;
     }
+    r.m.fi64 = ctx->fi + NSLOT * c.fi_off;
+    r.m.fj64 = ctx->fj + NSLOT * c.fj_off;
     const hc_species& s = c.cs.sp;
     r.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
     r.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
@@ -276,7 +278,7 @@ int build_layout(hc_context* ctx)
     const int nc = (int)ctx->cases.size();
     long long N = 0, nfi = 0, nfj = 0, br = 0;
     long long nt[NTILECFG] = {};
-    const TileShape* shapes = ctx->precision == HC_PRECISION_MIXED ? TILES32 : TILES64;
+    const TileShape* shapes = tile_shapes(ctx->precision == HC_PRECISION_MIXED, ctx->cs.law, ctx->cs.flux);
     for (auto& c : ctx->cases) {
         c.cell_off = N;
         c.fi_off = nfi;
@@ -874,7 +876,7 @@ int hc_set_precision(hc_context* ctx, int precision)
     HC_HIP(hipSetDevice(ctx->device));
     HC_HIP(hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
-    const TileShape* shapes = precision == HC_PRECISION_MIXED ? TILES32 : TILES64;
+    const TileShape* shapes = tile_shapes(precision == HC_PRECISION_MIXED, ctx->cs.law, ctx->cs.flux);
     const int nc = (int)ctx->cases.size();
     long long nt[NTILECFG] = {};
     for (auto& c : ctx->cases)

@@ -15,15 +15,13 @@
 
 #if HC_PREC == 64
 using RF = double;
-#define HC_TILES hc::TILES64
 #else
 using RF = float;
-#define HC_TILES hc::TILES32
 #endif
 
 template <int K> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
 {
-    constexpr hc::TileShape S = HC_TILES[K];
+    constexpr hc::TileShape S = hc::tile_shapes(HC_PREC == 32, HC_LAW, HC_FLUX)[K];
     hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt>), dim3(L.ntiles), dim3(S.nt), 0,
                        st, L);
     return hipGetLastError();

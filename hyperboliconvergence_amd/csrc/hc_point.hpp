@@ -32,6 +32,14 @@ enum LimiterId {
 };
 enum BcId { BC_PERIODIC = 0, BC_RIEMANN = 1, BC_ENTROPY = 2, BC_FIXED = 3, BC_WALL = 4 };
 
+// Mixed precision keeps the solution tile, face states and stored face fluxes in FP64 where the
+// flux splits into an FP64 central part and a dissipation proportional to the jump (Euler with
+// the central, Rusanov or Roe fluxes); SLAU and AUSM+up run their faces entirely in FP32.
+constexpr bool mixed_wide(int law, int flux)
+{
+    return law == 1 && (flux == 0 || flux == 1 || flux == 2 || flux == 5);
+}
+
 template <class R> struct NumLimits;
 template <> struct NumLimits<double> {
     HC_HD static double min() { return DBL_MIN; }
@@ -460,13 +468,14 @@ template <class R> HC_HD void entropyfix(const R* la, const R* ll, const R* lr, 
     }
 }
 
-// base/fluxes/roeFlux.ipp:39-95 (RoeFlux = CentralFlux + RoeDissipation).  FACEFRAME = true is the
+// base/fluxes/roeFlux.ipp:39-95 (RoeDissipation): fd = R |Lambda| L (dq) in the global frame for the
+// conserved jump dq, returns the Roe-state spectral radius times the area.  FACEFRAME = true is the
 // added variant that rotates the Roe-average velocity into the face frame like the jump; the
 // reference keeps it in the global frame (roeFlux.ipp:84-91).
 template <class R, bool FACEFRAME = false>
-HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
+HC_HD R roe_dissipation(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr, const R* dq,
+                        R* fd)
 {
-    const Flux4<R> c = euler_central(f, sl, sr);
     const EState<R> ravg = roe_average(sp, sl, sr);
     EState<R> rv = ravg;
     if constexpr (FACEFRAME) {
@@ -481,10 +490,7 @@ HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R
     wavespeeds(f, sl, wl);
     wavespeeds(f, sr, wr);
     entropyfix(wa, wl, wr, lam);
-    R cr[4], cl[4], dq[4], dqc[4];
-    euler_to_conserved(sr, cr);
-    euler_to_conserved(sl, cl);
-    for (int v = 0; v < 4; ++v) dq[v] = cr[v] - cl[v];
+    R dqc[4];
     rotate_to_face(f, dq, dqc);
     dqc[2] = dq[2];
     dqc[3] = dq[3];
@@ -492,17 +498,56 @@ HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R
     dcons_to_dprim(sp, rv, dqc, dqp);
     dprim_to_dchar(sp, rv, dqp, dqw);
     for (int v = 0; v < 4; ++v) dqw[v] = lam[v] * dqw[v];
-    R dqp2[4], dqc2[4], fd[4];
+    R dqp2[4], dqc2[4];
     dchar_to_dprim(sp, rv, dqw, dqp2);
     dprim_to_dcons(sp, rv, dqp2, dqc2);
     rotate_from_face(f, dqc2, fd);
     fd[2] = dqc2[2];
     fd[3] = dqc2[3];
+    return euler_specrad(f.n0, f.n1, ravg) * f.area;
+}
+
+// base/fluxes/roeFlux.ipp:39-95 (RoeFlux = CentralFlux + RoeDissipation)
+template <class R, bool FACEFRAME = false>
+HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
+{
+    const Flux4<R> c = euler_central(f, sl, sr);
+    R cr[4], cl[4], dq[4], fd[4];
+    euler_to_conserved(sr, cr);
+    euler_to_conserved(sl, cl);
+    for (int v = 0; v < 4; ++v) dq[v] = cr[v] - cl[v];
+    const R dl = roe_dissipation<R, FACEFRAME>(sp, f, sl, sr, dq, fd);
     const R m = R(-0.5) * f.area;
     Flux4<R> o;
     for (int v = 0; v < 4; ++v) o.f[v] = c.f[v] + fd[v] * m;
-    const R dl = euler_specrad(f.n0, f.n1, ravg) * f.area;
     o.lambda = hc_fmax(c.lambda, dl);
+    return o;
+}
+
+template <class R> HC_HD EState<float> state_f32(const EState<R>& s)
+{
+    return EState<float>{float(s.p), float(s.r), float(s.t), float(s.h), float(s.v2), float(s.a2), float(s.u), float(s.v)};
+}
+
+// Mixed-precision Roe: the central flux and the conserved jump in FP64 (O(1) quantities), the
+// dissipation R |Lambda| L (dq) in FP32 (proportional to the jump, so FP32 round-off stays
+// relative to the jump and not to the flux)
+template <bool FACEFRAME>
+HC_HD Flux4<double> euler_roe_mixed(const Species<float>& spf, const FaceG<double>& f, const EState<double>& sl,
+                                    const EState<double>& sr)
+{
+    const Flux4<double> c = euler_central(f, sl, sr);
+    double cr[4], cl[4];
+    euler_to_conserved(sr, cr);
+    euler_to_conserved(sl, cl);
+    float dq[4], fd[4];
+    for (int v = 0; v < 4; ++v) dq[v] = float(cr[v] - cl[v]);
+    const FaceG<float> ff{float(f.area), float(f.n0), float(f.n1)};
+    const float dl = roe_dissipation<float, FACEFRAME>(spf, ff, state_f32(sl), state_f32(sr), dq, fd);
+    const double m = -0.5 * f.area;
+    Flux4<double> o;
+    for (int v = 0; v < 4; ++v) o.f[v] = c.f[v] + double(fd[v]) * m;
+    o.lambda = hc_fmax(c.lambda, double(dl));
     return o;
 }
 
@@ -644,6 +689,35 @@ HC_HD Flux4<R> face_flux(const Species<R>& sp, int vscale, int pscale, const Fac
         else if constexpr (FLUX == FLUX_SLAU) return euler_slau(sp, f, sl, sr);
         else if constexpr (FLUX == FLUX_ROE_FACEFRAME) return euler_roe<R, true>(sp, f, sl, sr);
         else return euler_ausmpup(sp, vscale, pscale, f, sl, sr);
+    }
+}
+
+// Mixed-precision face flux on FP64 face states: the Roe family splits into an FP64 central flux
+// and an FP32 dissipation; Rusanov and central are cheap and stay FP64; SLAU and AUSM+up, which
+// do not split that way, run in FP32 on the rounded face states.
+template <int LAW, int BASIS, int FLUX>
+HC_HD Flux4<double> face_flux_mixed(const Species<double>& sp, const Species<float>& spf, int vscale, int pscale,
+                                    const FaceG<double>& f, const double* ql, const double* qr)
+{
+    if constexpr (LAW == LAW_EULER && (FLUX == FLUX_ROE || FLUX == FLUX_ROE_FACEFRAME)) {
+        const EState<double> sl = euler_state<double, BASIS>(sp, ql);
+        const EState<double> sr = euler_state<double, BASIS>(sp, qr);
+        return euler_roe_mixed<FLUX == FLUX_ROE_FACEFRAME>(spf, f, sl, sr);
+    } else if constexpr (FLUX == FLUX_RUSANOV || FLUX == FLUX_CENTRAL) {
+        return face_flux<double, LAW, BASIS, FLUX>(sp, vscale, pscale, f, ql, qr);
+    } else {
+        constexpr int NV = LawTraits<LAW>::nv;
+        float a[NV], b[NV];
+        for (int v = 0; v < NV; ++v) {
+            a[v] = float(ql[v]);
+            b[v] = float(qr[v]);
+        }
+        const FaceG<float> ff{float(f.area), float(f.n0), float(f.n1)};
+        const Flux4<float> F = face_flux<float, LAW, BASIS, FLUX>(spf, vscale, pscale, ff, a, b);
+        Flux4<double> o;
+        for (int v = 0; v < 4; ++v) o.f[v] = double(F.f[v]);
+        o.lambda = double(F.lambda);
+        return o;
     }
 }
 

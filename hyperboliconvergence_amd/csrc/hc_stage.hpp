@@ -26,6 +26,7 @@
 #endif
 
 #include <cstdint>
+#include <type_traits>
 #include <cstring>
 
 namespace hc {
@@ -54,6 +55,8 @@ template <class RF> struct MeshView {
     const double* ccy;
     const RF* fi;   // [NSLOT][nfi], i-face at node line l, row j: l*nj + j
     const RF* fj;   // [NSLOT][nfj], j-face at row i, node line l: i*(nj+1) + l
+    const double* fi64;   // the same slots in FP64 (area and normal of the mixed path)
+    const double* fj64;
 };
 
 // Runge-Kutta accumulation plan of one stage:
@@ -166,10 +169,18 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     static constexpr int NFJ = TI * (TJ + 1);
     static constexpr int NF = NFI + NFJ;
 
+    // Mixed precision (RF = float) with a splitting flux (mixed_wide): the solution tile, the face
+    // states, area and normal, the central flux and the stored face fluxes stay FP64 (O(1)
+    // quantities); the LSQ metric, the slopes, the limiter and the flux dissipation, which scale
+    // with the jumps, run in FP32.  Otherwise RF = float throughout the faces.
+    static constexpr bool MIXED = sizeof(RF) == sizeof(float) && mixed_wide(LAW, FLUX);
+    using QT = std::conditional_t<MIXED, double, RF>;   // solution tile
+    using FT = std::conditional_t<MIXED, double, RF>;   // face fluxes
+
     struct Smem {
-        RF q[NV][QH][QW];
+        QT q[NV][QH][QW];
         RF Q[NV][2][MH][MW];
-        RF F[NV + 1][NF];   // face fluxes and spectral radii, i-faces then j-faces
+        FT F[NV + 1][NF];   // face fluxes and spectral radii, i-faces then j-faces
         double red[NT];
         unsigned int flg[NT];
     };
@@ -189,7 +200,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             const int gi = wrapi(i0 - 2 + li, ni, a.m.per_i);
             const int gj = wrapi(j0 - 2 + lj, nj, a.m.per_j);
             const long long c = (long long)gi * nj + gj;
-            for (int v = 0; v < NV; ++v) s.q[v][li][lj] = RF(a.q_in[v * a.N + c]);
+            for (int v = 0; v < NV; ++v) s.q[v][li][lj] = QT(a.q_in[v * a.N + c]);
         }
     }
 
@@ -219,7 +230,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                 const RF dx0 = base[FS_DX0 * nf + fidx];
                 const RF dx1 = base[FS_DX1 * nf + fidx];
                 for (int v = 0; v < NV; ++v) {
-                    const RF dq = s.q[v][rr][cr] - s.q[v][rl][cl];
+                    const RF dq = RF(s.q[v][rr][cr] - s.q[v][rl][cl]);
                     acc[v][0] = acc[v][0] + dq * dx0;
                     acc[v][1] = acc[v][1] + dq * dx1;
                 }
@@ -244,48 +255,88 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     }
 
     // MUSCL face flux (spatial/muscl.h:10-87) with precomputed face geometry
-    HC_HD static Flux4<RF> muscl(const StageArgs<RF>& a, const RF* base, long long nf, long long fidx,
-                                 const RF* ql, const RF* qr, const RF (*Ql)[2], const RF (*Qr)[2])
+    HC_HD static Flux4<FT> muscl(const StageArgs<RF>& a, const RF* base, const double* base64, long long nf,
+                                 long long fidx, const QT* ql, const QT* qr, const RF (*Ql)[2], const RF (*Qr)[2])
     {
-        FaceG<RF> f;
-        f.area = base[FS_AREA * nf + fidx];
-        f.n0 = base[FS_N0 * nf + fidx];
-        f.n1 = base[FS_N1 * nf + fidx];
-        if (a.limiter == LIM_NOLIMIT1) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, ql, qr);
-        const RF dl = base[FS_DL * nf + fidx];
-        const RF dr = base[FS_DR * nf + fidx];
-        const RF s1 = base[FS_S1 * nf + fidx];
-        const RF dx0 = base[FS_DX0 * nf + fidx];
-        const RF dx1 = base[FS_DX1 * nf + fidx];
-        const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
-        const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
-        RF qlf[NV], qrf[NV];
-        for (int v = 0; v < NV; ++v) {
-            const RF dq = qr[v] - ql[v];
-            const RF dqdx = dq * s1;
-            const RF c0 = dq * dx0;
-            const RF c1 = dq * dx1;
-            const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
-            const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
-            const RF slope_l = limit(a.limiter, dqdx, gl);
-            const RF slope_r = limit(a.limiter, dqdx, gr);
-            qlf[v] = ql[v] + slope_l * dl;
-            qrf[v] = qr[v] - slope_r * dr;
+        if constexpr (MIXED) {
+            FaceG<double> f;
+            f.area = base64[FS_AREA * nf + fidx];
+            f.n0 = base64[FS_N0 * nf + fidx];
+            f.n1 = base64[FS_N1 * nf + fidx];
+            if (a.limiter == LIM_NOLIMIT1)
+                return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, ql, qr);
+            const RF n0 = RF(f.n0), n1 = RF(f.n1);
+            const RF dl = base[FS_DL * nf + fidx];
+            const RF dr = base[FS_DR * nf + fidx];
+            const RF s1 = base[FS_S1 * nf + fidx];
+            const RF dx0 = base[FS_DX0 * nf + fidx];
+            const RF dx1 = base[FS_DX1 * nf + fidx];
+            const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
+            const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
+            double qlf[NV], qrf[NV];
+            for (int v = 0; v < NV; ++v) {
+                const RF dq = RF(qr[v] - ql[v]);
+                const RF dqdx = dq * s1;
+                const RF c0 = dq * dx0;
+                const RF c1 = dq * dx1;
+                const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
+                const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
+                const RF slope_l = limit(a.limiter, dqdx, gl);
+                const RF slope_r = limit(a.limiter, dqdx, gr);
+                qlf[v] = ql[v] + double(slope_l * dl);
+                qrf[v] = qr[v] - double(slope_r * dr);
+            }
+            return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, qlf, qrf);
+        } else {
+            FaceG<RF> f;
+            f.area = base[FS_AREA * nf + fidx];
+            f.n0 = base[FS_N0 * nf + fidx];
+            f.n1 = base[FS_N1 * nf + fidx];
+            if (a.limiter == LIM_NOLIMIT1) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, ql, qr);
+            const RF dl = base[FS_DL * nf + fidx];
+            const RF dr = base[FS_DR * nf + fidx];
+            const RF s1 = base[FS_S1 * nf + fidx];
+            const RF dx0 = base[FS_DX0 * nf + fidx];
+            const RF dx1 = base[FS_DX1 * nf + fidx];
+            const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
+            const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
+            RF qlf[NV], qrf[NV];
+            for (int v = 0; v < NV; ++v) {
+                const RF dq = qr[v] - ql[v];
+                const RF dqdx = dq * s1;
+                const RF c0 = dq * dx0;
+                const RF c1 = dq * dx1;
+                const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
+                const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
+                const RF slope_l = limit(a.limiter, dqdx, gl);
+                const RF slope_r = limit(a.limiter, dqdx, gr);
+                qlf[v] = ql[v] + slope_l * dl;
+                qrf[v] = qr[v] - slope_r * dr;
+            }
+            return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, qlf, qrf);
         }
-        return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, qlf, qrf);
     }
 
     // non-periodic boundary face: ghost-cell flux or wall flux, in the orientation the
-    // cell accumulation expects (ghostCellLoop.h:133-224, boundaryFluxLoop.h:150-261)
-    HC_HD static Flux4<RF> boundary_face(const StageArgs<RF>& a, int bid, const RF* base, long long nf, long long fidx,
-                                         const RF* qint, long long cint, long long bref_row, unsigned int& flag)
+    // cell accumulation expects (ghostCellLoop.h:133-224, boundaryFluxLoop.h:150-261); the mixed
+    // path evaluates its (few) boundary faces in FP64
+    HC_HD static Flux4<FT> boundary_face(const StageArgs<RF>& a, int bid, const RF* base, const double* base64,
+                                         long long nf, long long fidx, const QT* qint, long long cint,
+                                         long long bref_row, unsigned int& flag)
     {
-        FaceG<RF> fres;
-        fres.area = base[FS_AREA * nf + fidx];
-        fres.n0 = base[FS_N0 * nf + fidx];
-        fres.n1 = base[FS_N1 * nf + fidx];
+        using BT = std::conditional_t<MIXED, double, RF>;
+        const BT* gb;
+        if constexpr (MIXED) gb = base64;
+        else gb = base;
+        const Species<BT>* sp;
+        if constexpr (MIXED) sp = &a.sp;
+        else sp = &a.spf;
+        FaceG<BT> fres;
+        fres.area = gb[FS_AREA * nf + fidx];
+        fres.n0 = gb[FS_N0 * nf + fidx];
+        fres.n1 = gb[FS_N1 * nf + fidx];
         // the update surface faces into the domain: reversed on boundaries 1 and 3
-        FaceG<RF> fupd = fres;
+        FaceG<BT> fupd = fres;
         if (bid == 1 || bid == 3) {
             fupd.n0 = -fres.n0;
             fupd.n1 = -fres.n1;
@@ -293,20 +344,20 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         // constant indices only: the arguments may live in registers
         const int type = bid == 0 ? a.bc[0] : bid == 1 ? a.bc[1] : bid == 2 ? a.bc[2] : a.bc[3];
         if constexpr (LAW == LAW_EULER) {
-            if (type == BC_WALL) return euler_wall_flux<RF, BASIS>(a.spf, fupd, qint);
+            if (type == BC_WALL) return euler_wall_flux<BT, BASIS>(*sp, fupd, qint);
         }
-        RF qref[NV], qg[NV];
-        for (int v = 0; v < NV; ++v) qref[v] = RF(a.bref[bref_row * NV + v]);
-        const RF fcx = base[FS_CX * nf + fidx];
-        const RF fcy = base[FS_CY * nf + fidx];
-        const RF ccx = RF(a.m.ccx[cint]);
-        const RF ccy = RF(a.m.ccy[cint]);
-        if (!ghost_value<RF, LAW, BASIS>(a.spf, type, fupd, fcx, fcy, ccx, ccy, qint, qref, qg)) flag |= ST_BCNORMAL;
-        if (bid == 0 || bid == 2) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, fres, qg, qint);
-        return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, fres, qint, qg);
+        BT qref[NV], qg[NV];
+        for (int v = 0; v < NV; ++v) qref[v] = BT(a.bref[bref_row * NV + v]);
+        const BT fcx = gb[FS_CX * nf + fidx];
+        const BT fcy = gb[FS_CY * nf + fidx];
+        const BT ccx = BT(a.m.ccx[cint]);
+        const BT ccy = BT(a.m.ccy[cint]);
+        if (!ghost_value<BT, LAW, BASIS>(*sp, type, fupd, fcx, fcy, ccx, ccy, qint, qref, qg)) flag |= ST_BCNORMAL;
+        if (bid == 0 || bid == 2) return face_flux<BT, LAW, BASIS, FLUX>(*sp, a.vscale, a.pscale, fres, qg, qint);
+        return face_flux<BT, LAW, BASIS, FLUX>(*sp, a.vscale, a.pscale, fres, qint, qg);
     }
 
-    HC_HD static void store(Smem& s, int k, const Flux4<RF>& F)
+    HC_HD static void store(Smem& s, int k, const Flux4<FT>& F)
     {
         for (int v = 0; v < NV; ++v) s.F[v][k] = F.f[v];
         s.F[NV][k] = F.lambda;
@@ -319,7 +370,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     HC_HD static void faces(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
     {
         const int ni = a.m.ni, nj = a.m.nj;
-        const RF* qs = &s.q[0][0][0];
+        const QT* qs = &s.q[0][0][0];
         const RF* Qs = &s.Q[0][0][0][0];
         for (int k = tid; k < NF; k += NT) {
             const bool isi = k < NFI;
@@ -332,10 +383,11 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             if (l > lmax || t >= (isi ? nj : ni)) continue;
             const bool interior = l >= 1 && l <= lmax - 1;
             const RF* base = isi ? a.m.fi : a.m.fj;
+            const double* base64 = isi ? a.m.fi64 : a.m.fj64;
             const long long nf = isi ? a.m.nfi : a.m.nfj;
             const int pr = (li + 2) * QW + lj + 2;        // right cell in the q tile
             const int pl = pr - (isi ? QW : 1);            // left cell
-            RF ql[NV], qr[NV];
+            QT ql[NV], qr[NV];
             if (interior || (isi ? a.m.per_i : a.m.per_j)) {
                 const int le = interior ? l : 0;
                 const long long fidx = isi ? (long long)le * nj + t : (long long)t * (nj + 1) + le;
@@ -350,7 +402,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                     Qr[v][0] = Qs[(2 * v) * MH * MW + mr];
                     Qr[v][1] = Qs[(2 * v + 1) * MH * MW + mr];
                 }
-                store(s, k, muscl(a, base, nf, fidx, ql, qr, Ql, Qr));
+                store(s, k, muscl(a, base, base64, nf, fidx, ql, qr, Ql, Qr));
             } else {
                 // the interior cell is right of node line 0 and left of the last node line
                 const bool first = l == 0;
@@ -361,7 +413,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                 const long long fidx = isi ? (long long)l * nj + t : (long long)t * (nj + 1) + l;
                 const long long cint = isi ? (long long)nc * nj + t : (long long)t * nj + nc;
                 const long long row = isi ? (first ? 0 : nj) + t : 2LL * nj + (first ? 0 : ni) + t;
-                store(s, k, boundary_face(a, bid, base, nf, fidx, ql, cint, row, flag));
+                store(s, k, boundary_face(a, bid, base, base64, nf, fidx, ql, cint, row, flag));
             }
         }
     }

@@ -2,6 +2,8 @@
 // launchers, the context's autotuner and the host emulation of the kernels).
 #pragma once
 
+#include "hc_point.hpp"
+
 namespace hc {
 
 struct TileShape {
@@ -10,12 +12,19 @@ struct TileShape {
 
 // Two shapes per precision; a context times both on its own cases and keeps the faster.
 // The face phase runs 2*ti*tj + ti + tj faces on nt lanes:
-//   FP64 4x32 on 256 lanes: 292 faces (one wave takes two), 128 cells per work-group; at
-//        4 work-groups per CU a 131072-cell grid is exactly one round of work-groups
-//   FP64 7x16 on 256 lanes: 247 faces, one per lane, every wave equally loaded
-//   mixed 8x32 on 256 lanes: 552 faces; mixed 7x32 on 256 lanes: 487 faces, two per lane
+//   4x32 on 256 lanes: 292 faces (one wave takes two), 128 cells per work-group; at 4
+//        work-groups per CU a 131072-cell grid is exactly one round of work-groups
+//   7x16 on 256 lanes: 247 faces, one per lane, every wave equally loaded
+// The mixed path of the Roe / Rusanov / central fluxes keeps its solution tile and face fluxes
+// in FP64 (mixed_wide), so it has the FP64 LDS budget and shapes; the all-FP32 faces of SLAU and
+// AUSM+up take larger tiles: 8x32 (552 faces) and 7x32 (487 faces, two per lane) on 256 lanes.
 constexpr int NTILECFG = 2;
 constexpr TileShape TILES64[NTILECFG] = {{4, 32, 256}, {7, 16, 256}};
 constexpr TileShape TILES32[NTILECFG] = {{8, 32, 256}, {7, 32, 256}};
+
+constexpr const TileShape* tile_shapes(bool mixed, int law, int flux)
+{
+    return (!mixed || mixed_wide(law, flux)) ? TILES64 : TILES32;
+}
 
 }   // namespace hc

@@ -37,6 +37,8 @@ int setup(AdjEmu& e, const hc_case_desc* cs, int ni, int nj, const double* nodes
     a.m.ccy = e.g.ccy.data();
     a.m.fi = e.g.fi.data();
     a.m.fj = e.g.fj.data();
+    a.m.fi64 = e.g.fi.data();
+    a.m.fj64 = e.g.fj.data();
     const hc_species& s = cs->sp;
     a.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
     a.vscale = cs->vscale;

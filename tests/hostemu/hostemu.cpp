@@ -52,6 +52,8 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
         a.m.fi = e.fi32.data();
         a.m.fj = e.fj32.data();
     }
+    a.m.fi64 = e.g.fi.data();
+    a.m.fj64 = e.g.fj.data();
     const hc_species& s = e.cs.sp;
     a.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
     a.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
@@ -144,8 +146,9 @@ int g_tile_cfg = 0;
 template <int K> void stage_shape(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
     constexpr TileShape S64 = TILES64[K], S32 = TILES32[K];
-    if (e.precision == HC_PRECISION_MIXED) dispatch_stage<float, S32.ti, S32.tj, S32.nt>(e.cs, a32);
-    else dispatch_stage<double, S64.ti, S64.tj, S64.nt>(e.cs, a64);
+    if (e.precision != HC_PRECISION_MIXED) dispatch_stage<double, S64.ti, S64.tj, S64.nt>(e.cs, a64);
+    else if (mixed_wide(e.cs.law, e.cs.flux)) dispatch_stage<float, S64.ti, S64.tj, S64.nt>(e.cs, a32);
+    else dispatch_stage<float, S32.ti, S32.tj, S32.nt>(e.cs, a32);
 }
 
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
