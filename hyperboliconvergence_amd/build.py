@@ -28,15 +28,22 @@ COMMON = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-I", CSRC, "-
 KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(6)]
 
 
-def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0)):
+def _mixed_wide(law, flux):
+    """The mixed path keeps FP64 face states for these (hc_point.hpp mixed_wide)."""
+    return law == 1 and flux in (0, 1, 2, 5)
+
+
+def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0), mixed_fast_div="auto"):
     objs = []
     bdir = build_dir or BUILD
     for (law, basis, flux) in KERNELS:
         for prec in (64, 32):
             # FP64 objects: no contraction (bit-faithful); mixed objects: contraction and the
             # hardware FP32 reciprocal / square root instead of the correctly rounded sequences
+            fast_div = {"all": True, "none": False}.get(mixed_fast_div, not _mixed_wide(law, flux))
             fpflags = (["-ffp-contract=off"] if prec == 64 else
-                       [f"-ffp-contract={mixed_contract}", "-fno-hip-fp32-correctly-rounded-divide-sqrt"])
+                       [f"-ffp-contract={mixed_contract}"] +
+                       (["-fno-hip-fp32-correctly-rounded-divide-sqrt"] if fast_div else []))
             wpe = waves_per_eu[0] if prec == 64 else waves_per_eu[1]
             if wpe:
                 fpflags.append(f"-DHC_STAGE_WAVES_PER_EU={wpe}")
@@ -59,12 +66,15 @@ def _run(job):
     return out
 
 
-def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0)):
+def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0),
+          mixed_fast_div="auto"):
     """Compile every HIP source and link libhcamd.so; always recompiles.
-    waves_per_eu: occupancy hint (waves per SIMD) of the FP64 / mixed stage kernels, 0 = none."""
+    waves_per_eu: occupancy hint (waves per SIMD) of the FP64 / mixed stage kernels, 0 = none.
+    mixed_fast_div: hardware FP32 reciprocal / square root in the mixed objects: "auto" where the
+    faces run entirely in FP32 (SLAU, AUSM+up), "all", or "none"."""
     bdir = build_dir or BUILD
     os.makedirs(bdir, exist_ok=True)
-    work = _jobs(mixed_contract, bdir, waves_per_eu)
+    work = _jobs(mixed_contract, bdir, waves_per_eu, mixed_fast_div)
     n = jobs or min(8, os.cpu_count() or 4)
     with ThreadPoolExecutor(max_workers=n) as ex:
         objs = list(ex.map(_run, work))
@@ -85,7 +95,9 @@ if __name__ == "__main__":
     ap.add_argument("--out", default=LIB)
     ap.add_argument("--mixed-contract", default="fast", choices=["fast", "on", "off"])
     ap.add_argument("--waves-per-eu", type=int, nargs=2, default=(0, 0), metavar=("FP64", "MIXED"))
+    ap.add_argument("--mixed-fast-div", default="auto", choices=["auto", "all", "none"])
     a = ap.parse_args()
     build(a.jobs, verbose=True, lib=a.out, mixed_contract=a.mixed_contract,
-          build_dir=None if a.out == LIB else os.path.join(BUILD, "alt"), waves_per_eu=tuple(a.waves_per_eu))
+          build_dir=None if a.out == LIB else os.path.join(BUILD, "alt"), waves_per_eu=tuple(a.waves_per_eu),
+          mixed_fast_div=a.mixed_fast_div)
     sys.exit(0)

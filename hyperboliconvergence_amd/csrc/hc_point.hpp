@@ -465,6 +465,11 @@ template <class R> HC_HD void entropyfix(const R* la, const R* ll, const R* lr, 
         const R le = eps * hc_fmax(R(0.), hc_fmax(la[i] - ll[i], lr[i] - la[i]));
         const R laa = hc_fabs(la[i]);
         s[i] = (laa > le) ? laa : R(0.5) * (le + laa * laa / le);
+        // FP32 (the mixed path's dissipation): a sonic Roe state (laa = 0) inside a compression
+        // (le = 0) is frequent enough at 24 bits to hit 0/0; the formula's limit there is laa
+        if constexpr (sizeof(R) == sizeof(float)) {
+            if (!(le > R(0))) s[i] = laa;
+        }
     }
 }
 

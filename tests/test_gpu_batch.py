@@ -161,3 +161,23 @@ def test_batch_rejects_cases_with_different_schedules():
     with pytest.raises(hca.HCError):
         b.add_case(c["nodes"], c["case"])
     b.close()
+
+
+def test_mixed_transonic_case_stays_finite_and_close_to_fp64():
+    """Regression: a Mach 0.77 cylinder (supersonic pocket, sonic Roe states inside compressions)
+    once produced 0/0 in the FP32 entropy fix of the mixed path after ~160 steps."""
+    import hyperboliconvergence_amd as hca
+    from hyperboliconvergence_amd import cases
+    r = cases.batch_case_table()[12]
+    cfg = cases.config_cylinder(r["nth"], r["nr"], r["mach"], r["alpha_deg"], flux=hca.FLUX_ROE_FACEFRAME)
+    out = []
+    for prec in (0, 1):
+        with _single(cfg, prec) as ctx:
+            ctx.set_graph_steps(50)
+            ctx.step(300)
+            out.append(ctx.get_state())
+            assert ctx.status() == 0, prec
+    qc = [cases.prim_to_cons(q) for q in out]
+    err = max(np.max(np.abs(qc[1][..., v] - qc[0][..., v])) / np.max(np.abs(qc[0][..., v])) for v in range(4))
+    print("mixed vs fp64 after 300 steps: %.3e" % err)
+    assert err <= 1e-6
