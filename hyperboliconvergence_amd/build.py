@@ -40,7 +40,7 @@ def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0), mixed_fast
         for prec in (64, 32):
             # FP64 objects: no contraction (bit-faithful); mixed objects: contraction and the
             # hardware FP32 reciprocal / square root instead of the correctly rounded sequences
-            fast_div = {"all": True, "none": False}.get(mixed_fast_div, not _mixed_wide(law, flux))
+            fast_div = {"all": True, "none": False}.get(mixed_fast_div, True)
             fpflags = (["-ffp-contract=off"] if prec == 64 else
                        [f"-ffp-contract={mixed_contract}"] +
                        (["-fno-hip-fp32-correctly-rounded-divide-sqrt"] if fast_div else []))
@@ -70,8 +70,8 @@ def build(jobs=None, verbose=False, lib=LIB, mixed_contract="fast", build_dir=No
           mixed_fast_div="auto"):
     """Compile every HIP source and link libhcamd.so; always recompiles.
     waves_per_eu: occupancy hint (waves per SIMD) of the FP64 / mixed stage kernels, 0 = none.
-    mixed_fast_div: hardware FP32 reciprocal / square root in the mixed objects: "auto" where the
-    faces run entirely in FP32 (SLAU, AUSM+up), "all", or "none"."""
+    mixed_fast_div: hardware FP32 reciprocal / square root in the mixed objects ("auto" = "all"),
+    or "none" for the correctly rounded sequences."""
     bdir = build_dir or BUILD
     os.makedirs(bdir, exist_ok=True)
     work = _jobs(mixed_contract, bdir, waves_per_eu, mixed_fast_div)
