@@ -34,7 +34,7 @@ C_ABI_SYMBOLS = (
     "hc_set_tiles", "hc_get_tiles",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
 )
-NTILECFG = 2
+NTILECFG = 3
 
 
 class Species(C.Structure):
@@ -220,7 +220,8 @@ class Context:
         return out
 
     def set_tiles(self, config):
-        """-1: time every tile shape on the first steps and keep the fastest; 0, 1: fixed shape."""
+        """-1: time every tile configuration on the first steps and keep the fastest; 0 .. NTILECFG-1:
+        fixed configuration (shape, centre staging)."""
         self._check(self._L.hc_set_tiles(self._h, int(config)), "hc_set_tiles")
 
     def tiles(self):

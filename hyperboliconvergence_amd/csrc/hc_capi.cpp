@@ -255,6 +255,10 @@ template <class RF> void fill_rec(const hc_context* ctx, const CaseHost& c, int 
     }
     r.m.fi64 = ctx->fi + NSLOT * c.fi_off;
     r.m.fj64 = ctx->fj + NSLOT * c.fj_off;
+    for (int d = 0; d < 2; ++d) {
+        r.m.period_i[d] = c.cs.period_i[d];
+        r.m.period_j[d] = c.cs.period_j[d];
+    }
     const hc_species& s = c.cs.sp;
     r.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
     r.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
@@ -681,7 +685,7 @@ int get_time(hc_context* ctx, int k, double* t)
 }
 
 // steps per shape and repetitions of the tile-shape trial
-constexpr int TUNE_STEPS = 1, TUNE_REPS = 2;
+constexpr int TUNE_STEPS = 1, TUNE_REPS = 3;
 
 // time every tile shape on real steps (the shapes give bit-identical results) and keep the fastest
 int autotune(hc_context* ctx)

@@ -22,18 +22,19 @@ using RF = float;
 template <int K> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
 {
     constexpr hc::TileShape S = hc::tile_shapes(HC_PREC == 32, HC_LAW, HC_FLUX)[K];
-    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt>), dim3(L.ntiles), dim3(S.nt), 0,
-                       st, L);
+    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst>), dim3(L.ntiles),
+                       dim3(S.nt), 0, st, L);
     return hipGetLastError();
 }
 
 hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::LaunchArgs<RF>& L, int cfg, hipStream_t st)
 {
-    static_assert(hc::NTILECFG == 2, "one launch_shape per tile shape");
+    static_assert(hc::NTILECFG == 3, "one launch_shape per tile configuration");
     if (L.ntiles <= 0) return hipSuccess;
     switch (cfg) {
     case 0: return launch_shape<0>(L, st);
     case 1: return launch_shape<1>(L, st);
+    case 2: return launch_shape<2>(L, st);
     default: return hipErrorInvalidValue;
     }
 }

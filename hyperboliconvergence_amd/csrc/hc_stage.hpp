@@ -57,6 +57,8 @@ template <class RF> struct MeshView {
     const RF* fj;   // [NSLOT][nfj], j-face at row i, node line l: i*(nj+1) + l
     const double* fi64;   // the same slots in FP64 (area and normal of the mixed path)
     const double* fj64;
+    double period_i[2];   // translation from a cell to its periodic image across boundary 1
+    double period_j[2];   // ... across boundary 3
 };
 
 // Runge-Kutta accumulation plan of one stage:
@@ -158,7 +160,7 @@ HC_HD int xcd_tile(int b, int ntiles)
 
 // NTHR threads per work-group, at least one per cell: the extra lanes share the load, metric and
 // face phases, so more waves are resident per SIMD for the same LDS tile
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ> struct StageTile {
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ, int CST = 0> struct StageTile {
     static constexpr int NV = LawTraits<LAW>::nv;
     static constexpr int NC = TI * TJ;   // cells of the tile
     static constexpr int NT = NTHR;      // threads of the work-group
@@ -177,12 +179,18 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     using QT = std::conditional_t<MIXED, double, RF>;   // solution tile
     using FT = std::conditional_t<MIXED, double, RF>;   // face fluxes
 
+    // Centre staging (FP64 and wide mixed kernels): the cell centres of the q tile sit in LDS and
+    // the centre differences dx_c of the metric edges and faces are formed there, with the
+    // host geometry's operations (c_R - c_L, and c_R - (c_L - period) across a periodic wrap),
+    // instead of being read as two face slots per edge.
+    static constexpr bool CSTAGE = CST != 0 && (sizeof(RF) == sizeof(double) || MIXED);
+    static constexpr int CH = CSTAGE ? QH : 1, CW = CSTAGE ? QW : 1;
+
     struct Smem {
         QT q[NV][QH][QW];
         RF Q[NV][2][MH][MW];
-        FT F[NV + 1][NF];   // face fluxes and spectral radii, i-faces then j-faces
-        double red[NT];
-        unsigned int flg[NT];
+        FT F[NV + 1][NF];       // face fluxes and spectral radii, i-faces then j-faces
+        double gc[2][CH][CW];   // cell centres of the q tile
     };
 
     HC_HD static int wrapi(int g, int n, int per)
@@ -201,7 +209,28 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             const int gj = wrapi(j0 - 2 + lj, nj, a.m.per_j);
             const long long c = (long long)gi * nj + gj;
             for (int v = 0; v < NV; ++v) s.q[v][li][lj] = QT(a.q_in[v * a.N + c]);
+            if constexpr (CSTAGE) {
+                s.gc[0][li][lj] = a.m.ccx[c];
+                s.gc[1][li][lj] = a.m.ccy[c];
+            }
         }
+    }
+
+    // centre difference of the cells at q-tile positions pl, pr: c_R - c_L, or across a periodic
+    // wrap c_R - (c_L - period) (the host geometry's wrap-face operations)
+    HC_HD static void centre_diff(const StageArgs<RF>& a, const Smem& s, int pl, int pr, bool wrap, bool isi,
+                                  double& d0, double& d1)
+    {
+        const double* gcx = &s.gc[0][0][0];
+        const double* gcy = &s.gc[1][0][0];
+        double lx = gcx[pl], ly = gcy[pl];
+        if (wrap) {
+            const double* per = isi ? a.m.period_i : a.m.period_j;
+            lx = lx - per[0];
+            ly = ly - per[1];
+        }
+        d0 = gcx[pr] - lx;
+        d1 = gcy[pr] - ly;
     }
 
     // ---- phase 2: LSQ solution metric of tile cells and the 1-cell ring ----
@@ -226,26 +255,34 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                 acc[v][0] = RF(0);
                 acc[v][1] = RF(0);
             }
-            auto edge = [&](int rl, int cl, int rr, int cr, const RF* base, long long nf, long long fidx) {
-                const RF dx0 = base[FS_DX0 * nf + fidx];
-                const RF dx1 = base[FS_DX1 * nf + fidx];
+            auto edge = [&](int rl, int cl, int rr, int cr, const RF* base, long long nf, long long fidx, bool wrap) {
+                RF dx0, dx1;
+                if constexpr (CSTAGE) {
+                    double d0, d1;
+                    centre_diff(a, s, rl * QW + cl, rr * QW + cr, wrap, cl == cr, d0, d1);
+                    dx0 = RF(d0);
+                    dx1 = RF(d1);
+                } else {
+                    dx0 = base[FS_DX0 * nf + fidx];
+                    dx1 = base[FS_DX1 * nf + fidx];
+                }
                 for (int v = 0; v < NV; ++v) {
                     const RF dq = RF(s.q[v][rr][cr] - s.q[v][rl][cl]);
                     acc[v][0] = acc[v][0] + dq * dx0;
                     acc[v][1] = acc[v][1] + dq * dx1;
                 }
             };
-            if (gi > 0) edge(r - 1, c, r, c, fi, nfi, (long long)gi * nj + gj);
-            if (gi < ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)(gi + 1) * nj + gj);
-            if (gj > 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1) + gj);
-            if (gj < nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1) + gj + 1);
+            if (gi > 0) edge(r - 1, c, r, c, fi, nfi, (long long)gi * nj + gj, false);
+            if (gi < ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)(gi + 1) * nj + gj, false);
+            if (gj > 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1) + gj, false);
+            if (gj < nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1) + gj + 1, false);
             if (a.m.per_i) {
-                if (gi == 0) edge(r - 1, c, r, c, fi, nfi, (long long)gj);
-                if (gi == ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)gj);
+                if (gi == 0) edge(r - 1, c, r, c, fi, nfi, (long long)gj, true);
+                if (gi == ni - 1) edge(r, c, r + 1, c, fi, nfi, (long long)gj, true);
             }
             if (a.m.per_j) {
-                if (gj == 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1));
-                if (gj == nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1));
+                if (gj == 0) edge(r, c - 1, r, c, fj, nfj, (long long)gi * (nj + 1), true);
+                if (gj == nj - 1) edge(r, c, r, c + 1, fj, nfj, (long long)gi * (nj + 1), true);
             }
             for (int v = 0; v < NV; ++v) {
                 s.Q[v][0][li][lj] = acc[v][0];
@@ -255,8 +292,10 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     }
 
     // MUSCL face flux (spatial/muscl.h:10-87) with precomputed face geometry
+    // (cx0, cx1): the centre difference dx_c formed in LDS (centre staging), else read from the slots
     HC_HD static Flux4<FT> muscl(const StageArgs<RF>& a, const RF* base, const double* base64, long long nf,
-                                 long long fidx, const QT* ql, const QT* qr, const RF (*Ql)[2], const RF (*Qr)[2])
+                                 long long fidx, const QT* ql, const QT* qr, const RF (*Ql)[2], const RF (*Qr)[2],
+                                 double cx0, double cx1)
     {
         if constexpr (MIXED) {
             FaceG<double> f;
@@ -269,8 +308,8 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             const RF dl = base[FS_DL * nf + fidx];
             const RF dr = base[FS_DR * nf + fidx];
             const RF s1 = base[FS_S1 * nf + fidx];
-            const RF dx0 = base[FS_DX0 * nf + fidx];
-            const RF dx1 = base[FS_DX1 * nf + fidx];
+            const RF dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
+            const RF dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
             const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
             const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
             double qlf[NV], qrf[NV];
@@ -296,8 +335,8 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             const RF dl = base[FS_DL * nf + fidx];
             const RF dr = base[FS_DR * nf + fidx];
             const RF s1 = base[FS_S1 * nf + fidx];
-            const RF dx0 = base[FS_DX0 * nf + fidx];
-            const RF dx1 = base[FS_DX1 * nf + fidx];
+            const RF dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
+            const RF dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
             const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
             const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
             RF qlf[NV], qrf[NV];
@@ -402,7 +441,9 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                     Qr[v][0] = Qs[(2 * v) * MH * MW + mr];
                     Qr[v][1] = Qs[(2 * v + 1) * MH * MW + mr];
                 }
-                store(s, k, muscl(a, base, base64, nf, fidx, ql, qr, Ql, Qr));
+                double cx0 = 0.0, cx1 = 0.0;
+                if constexpr (CSTAGE) centre_diff(a, s, pl, pr, !interior, isi, cx0, cx1);
+                store(s, k, muscl(a, base, base64, nf, fidx, ql, qr, Ql, Qr, cx0, cx1));
             } else {
                 // the interior cell is right of node line 0 and left of the last node line
                 const bool first = l == 0;
@@ -419,16 +460,15 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     }
 
     // ---- phase 4: owner-computes residual accumulation and the stage update ----
-    HC_HD static void cells(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
+    // red: the cell's lambda/vol for the global-dt reduction of stage 0 (0 otherwise)
+    HC_HD static void cells(const StageArgs<RF>& a, const Smem& s, int i0, int j0, int tid, unsigned int& flag,
+                            double& red)
     {
         const int ni = a.m.ni, nj = a.m.nj;
         const int li = tid / TJ, lj = tid % TJ;
         const int i = i0 + li, j = j0 + lj;
-        s.red[tid] = 0.0;
-        if (tid >= NC || i >= ni || j >= nj) {
-            s.flg[tid] = flag;
-            return;
-        }
+        red = 0.0;
+        if (tid >= NC || i >= ni || j >= nj) return;
         double R[NV];
         double lam = 0.0;
         for (int v = 0; v < NV; ++v) R[v] = 0.0;
@@ -466,7 +506,6 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         if (a.mode == MODE_RESIDUAL) {
             for (int v = 0; v < NV; ++v) a.res_out[c * (NV + 1) + v] = R[v];
             a.res_out[c * (NV + 1) + NV] = lam;
-            s.flg[tid] = flag;
             return;
         }
         // Runge-Kutta accumulation (rungeKutta.h:176-197)
@@ -489,8 +528,7 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         const double vol = a.m.cvol[c];
         if (a.mode == MODE_GLOBAL_STAGE0) {
             for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
-            s.red[tid] = lam / vol;   // spectralRadius.h:399-401
-            s.flg[tid] = flag;
+            red = lam / vol;   // spectralRadius.h:399-401
             return;
         }
         double fac;
@@ -510,7 +548,6 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
             fac = (p.beta * a.cfl) / l0;
         }
         update_cell(a, c, Rt, fac, flag);
-        s.flg[tid] = flag;
     }
 
     // q_out = S( C(q0) + R_tot*fac )   (spatial/eulerForwardUpdate.h:274-283)
@@ -655,10 +692,10 @@ __device__ inline unsigned int wave_or(unsigned int v)
 #define HC_STAGE_OCCUPANCY
 #endif
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR>
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST>
 __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF> L)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
@@ -676,10 +713,11 @@ __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF
     __syncthreads();
     T::faces(a, s, i0, j0, tid, flag);
     __syncthreads();
-    T::cells(a, s, i0, j0, tid, flag);
+    double red;
+    T::cells(a, s, i0, j0, tid, flag, red);
     // work-group reductions: max(lambda/vol) for global-dt stage 0 and the status bits
-    const double m = wave_max(s.red[tid]);
-    const unsigned int f = wave_or(s.flg[tid]);
+    const double m = wave_max(red);
+    const unsigned int f = wave_or(flag);
     if ((tid & 63) == 0) {
         if (a.mode == MODE_GLOBAL_STAGE0) atomicMax(a.maxslot, dbl_bits(m));
         if (f) atomicOr(a.status, f);

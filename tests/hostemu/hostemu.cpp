@@ -54,6 +54,10 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
     }
     a.m.fi64 = e.g.fi.data();
     a.m.fj64 = e.g.fj.data();
+    for (int d = 0; d < 2; ++d) {
+        a.m.period_i[d] = e.cs.period_i[d];
+        a.m.period_j[d] = e.cs.period_j[d];
+    }
     const hc_species& s = e.cs.sp;
     a.sp = Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
     a.spf = Species<RF>{RF(s.gamma), RF(s.minf), RF(s.lref), RF(s.nu), RF(s.pr), RF(s.dt), RF(s.R), RF(s.gamma1)};
@@ -75,9 +79,9 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
     return a;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT> void run_stage(const StageArgs<RF>& a)
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST> void run_stage(const StageArgs<RF>& a)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST>;
     const int tiles_i = (a.m.ni + TI - 1) / TI, tiles_j = (a.m.nj + TJ - 1) / TJ;
     const int ntiles = tiles_i * tiles_j;
     auto sp = std::make_unique<typename T::Smem>();
@@ -90,12 +94,13 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT> void r
         for (int tid = 0; tid < T::NT; ++tid) T::load(a, s, i0, j0, tid);
         for (int tid = 0; tid < T::NT; ++tid) T::qmetric(a, s, i0, j0, tid);
         for (int tid = 0; tid < T::NT; ++tid) T::faces(a, s, i0, j0, tid, flag[tid]);
-        for (int tid = 0; tid < T::NT; ++tid) T::cells(a, s, i0, j0, tid, flag[tid]);
+        std::vector<double> red(T::NT);
+        for (int tid = 0; tid < T::NT; ++tid) T::cells(a, s, i0, j0, tid, flag[tid], red[tid]);
         double m = 0.0;
         unsigned int f = 0;
         for (int tid = 0; tid < T::NT; ++tid) {
-            m = fmax(m, s.red[tid]);
-            f |= s.flg[tid];
+            m = fmax(m, red[tid]);
+            f |= flag[tid];
         }
         if (a.mode == MODE_GLOBAL_STAGE0 && dbl_bits(m) > *a.maxslot) *a.maxslot = dbl_bits(m);
         *a.status |= f;
@@ -110,24 +115,24 @@ template <int LAW, int BASIS, class RF> void run_update(const StageArgs<RF>& a)
     *a.status |= flag;
 }
 
-template <class RF, int TI, int TJ, int NT> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
+template <class RF, int TI, int TJ, int NT, int CST> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
 {
     const int key = cs.law * 100 + cs.basis * 10 + cs.flux;
     switch (key) {
-    case 0: run_stage<0, 0, 0, RF, TI, TJ, NT>(a); break;
-    case 1: run_stage<0, 0, 1, RF, TI, TJ, NT>(a); break;
-    case 100: run_stage<1, 0, 0, RF, TI, TJ, NT>(a); break;
-    case 101: run_stage<1, 0, 1, RF, TI, TJ, NT>(a); break;
-    case 102: run_stage<1, 0, 2, RF, TI, TJ, NT>(a); break;
-    case 103: run_stage<1, 0, 3, RF, TI, TJ, NT>(a); break;
-    case 104: run_stage<1, 0, 4, RF, TI, TJ, NT>(a); break;
-    case 105: run_stage<1, 0, 5, RF, TI, TJ, NT>(a); break;
-    case 110: run_stage<1, 1, 0, RF, TI, TJ, NT>(a); break;
-    case 111: run_stage<1, 1, 1, RF, TI, TJ, NT>(a); break;
-    case 112: run_stage<1, 1, 2, RF, TI, TJ, NT>(a); break;
-    case 113: run_stage<1, 1, 3, RF, TI, TJ, NT>(a); break;
-    case 114: run_stage<1, 1, 4, RF, TI, TJ, NT>(a); break;
-    case 115: run_stage<1, 1, 5, RF, TI, TJ, NT>(a); break;
+    case 0: run_stage<0, 0, 0, RF, TI, TJ, NT, CST>(a); break;
+    case 1: run_stage<0, 0, 1, RF, TI, TJ, NT, CST>(a); break;
+    case 100: run_stage<1, 0, 0, RF, TI, TJ, NT, CST>(a); break;
+    case 101: run_stage<1, 0, 1, RF, TI, TJ, NT, CST>(a); break;
+    case 102: run_stage<1, 0, 2, RF, TI, TJ, NT, CST>(a); break;
+    case 103: run_stage<1, 0, 3, RF, TI, TJ, NT, CST>(a); break;
+    case 104: run_stage<1, 0, 4, RF, TI, TJ, NT, CST>(a); break;
+    case 105: run_stage<1, 0, 5, RF, TI, TJ, NT, CST>(a); break;
+    case 110: run_stage<1, 1, 0, RF, TI, TJ, NT, CST>(a); break;
+    case 111: run_stage<1, 1, 1, RF, TI, TJ, NT, CST>(a); break;
+    case 112: run_stage<1, 1, 2, RF, TI, TJ, NT, CST>(a); break;
+    case 113: run_stage<1, 1, 3, RF, TI, TJ, NT, CST>(a); break;
+    case 114: run_stage<1, 1, 4, RF, TI, TJ, NT, CST>(a); break;
+    case 115: run_stage<1, 1, 5, RF, TI, TJ, NT, CST>(a); break;
     default: break;
     }
 }
@@ -146,15 +151,16 @@ int g_tile_cfg = 0;
 template <int K> void stage_shape(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
     constexpr TileShape S64 = TILES64[K], S32 = TILES32[K];
-    if (e.precision != HC_PRECISION_MIXED) dispatch_stage<double, S64.ti, S64.tj, S64.nt>(e.cs, a64);
-    else if (mixed_wide(e.cs.law, e.cs.flux)) dispatch_stage<float, S64.ti, S64.tj, S64.nt>(e.cs, a32);
-    else dispatch_stage<float, S32.ti, S32.tj, S32.nt>(e.cs, a32);
+    if (e.precision != HC_PRECISION_MIXED) dispatch_stage<double, S64.ti, S64.tj, S64.nt, S64.cst>(e.cs, a64);
+    else if (mixed_wide(e.cs.law, e.cs.flux)) dispatch_stage<float, S64.ti, S64.tj, S64.nt, S64.cst>(e.cs, a32);
+    else dispatch_stage<float, S32.ti, S32.tj, S32.nt, S32.cst>(e.cs, a32);
 }
 
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
-    static_assert(NTILECFG == 2, "one stage_shape per tile shape");
-    if (g_tile_cfg == 1) stage_shape<1>(e, a64, a32);
+    static_assert(NTILECFG == 3, "one stage_shape per tile configuration");
+    if (g_tile_cfg == 2) stage_shape<2>(e, a64, a32);
+    else if (g_tile_cfg == 1) stage_shape<1>(e, a64, a32);
     else stage_shape<0>(e, a64, a32);
 }
 

@@ -15,7 +15,7 @@ from conftest import rel_err
 G = 1.4
 
 
-@pytest.fixture(autouse=True, params=[0, 1], ids=["tiles0", "tiles1"])
+@pytest.fixture(autouse=True, params=[0, 1, 2], ids=["tiles0", "tiles1", "tiles2"])
 def tile_config(request):
     """Every test runs with each tile shape of the kernels (the results must not depend on it)."""
     pyemu.set_tile_config(request.param)
