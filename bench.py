@@ -159,13 +159,14 @@ def cpu_baseline(cfg, budget_s=12.0):
                       f"steady_clock around the time loop"}
 
 
-def traffic_from_profiles(workload, precision):
-    """HBM bytes per stage-kernel launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/)."""
+def traffic_from_profiles(workload, precision, tile_cfg):
+    """HBM bytes per stage-kernel launch of the tile configuration in use, measured with rocprofv3
+    FETCH_SIZE/WRITE_SIZE (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(f"{workload}:{'mixed' if precision else 'fp64'}")
+        return d.get(f"{workload}:{'mixed' if precision else 'fp64'}:{tile_cfg}")
     except (OSError, ValueError):
         return None
 
@@ -288,7 +289,7 @@ def main_batch(a):
     bpc = 112.0
     achieved = bpc * Ntot / (h["kernel_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("c4", head),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("c4", head, h["tiles"]),
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bpc, "kernel_us": h["kernel_ms"] * 1e3}
     cpu = None if a.no_cpu_baseline else cpu_baseline_batch(cfgs)
     out = {
@@ -503,7 +504,7 @@ def main():
     alg_bytes = bytes_per_cell_stage * N
     achieved = alg_bytes / (h["kernel_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload, head),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload, head, h["tiles"]),
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bytes_per_cell_stage,
                 "kernel_us": h["kernel_ms"] * 1e3, "kernel_us_eager_events": h["launch_ms"] * 1e3,
                 "update_kernel_us": h["upd_ms"] * 1e3 if cs.dt_mode == 0 else None}
