@@ -39,11 +39,14 @@ def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0), mixed_fast
     for (law, basis, flux) in KERNELS:
         for prec in (64, 32):
             # FP64 objects: no contraction (bit-faithful); mixed objects: contraction and the
-            # hardware FP32 reciprocal / square root instead of the correctly rounded sequences
+            # hardware FP32 reciprocal / square root instead of the correctly rounded sequences,
+            # with FP32 denormals flushed (no range reduction around them; the mixed path's FP32
+            # quantities are jumps, slopes and dissipation terms far above FLT_MIN)
             fast_div = {"all": True, "none": False}.get(mixed_fast_div, True)
             fpflags = (["-ffp-contract=off"] if prec == 64 else
                        [f"-ffp-contract={mixed_contract}"] +
-                       (["-fno-hip-fp32-correctly-rounded-divide-sqrt"] if fast_div else []))
+                       (["-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero"]
+                        if fast_div else []))
             wpe = waves_per_eu[0] if prec == 64 else waves_per_eu[1]
             if wpe:
                 fpflags.append(f"-DHC_STAGE_WAVES_PER_EU={wpe}")

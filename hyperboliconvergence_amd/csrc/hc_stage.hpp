@@ -193,9 +193,16 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         double gc[2][CH][CW];   // cell centres of the q tile
     };
 
+    // periodic wrap or clamp of a halo index; on grids larger than the halo reach the wrap is
+    // one step of n, and the integer division runs only on lanes of smaller grids
     HC_HD static int wrapi(int g, int n, int per)
     {
-        if (per) return ((g % n) + n) % n;
+        if (per) {
+            if (g < 0) g += n;
+            else if (g >= n) g -= n;
+            if ((unsigned)g >= (unsigned)n) g = ((g % n) + n) % n;
+            return g;
+        }
         return g < 0 ? 0 : (g >= n ? n - 1 : g);
     }
 
