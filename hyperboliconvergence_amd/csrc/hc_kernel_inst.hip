@@ -22,7 +22,7 @@ using RF = float;
 template <int K> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
 {
     constexpr hc::TileShape S = hc::tile_shapes(HC_PREC == 32, HC_LAW, HC_FLUX)[K];
-    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst>), dim3(L.ntiles),
+    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst, S.gpf>), dim3(L.ntiles),
                        dim3(S.nt), 0, st, L);
     return hipGetLastError();
 }

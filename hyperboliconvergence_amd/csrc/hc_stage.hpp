@@ -160,7 +160,8 @@ HC_HD int xcd_tile(int b, int ntiles)
 
 // NTHR threads per work-group, at least one per cell: the extra lanes share the load, metric and
 // face phases, so more waves are resident per SIMD for the same LDS tile
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ, int CST = 0> struct StageTile {
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ, int CST = 0, int GPF = 0>
+struct StageTile {
     static constexpr int NV = LawTraits<LAW>::nv;
     static constexpr int NC = TI * TJ;   // cells of the tile
     static constexpr int NT = NTHR;      // threads of the work-group
@@ -298,66 +299,78 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
         }
     }
 
+    // Geometry of one interior or periodic face: its slots (area and normal in FP64 for the
+    // mixed path) and the centre difference
+    using GT = std::conditional_t<MIXED, double, RF>;
+    struct Geo {
+        GT area, n0, n1;
+        RF dl, dr, s1, dx0, dx1, fl[3], fr[3];
+    };
+
+    // (cx0, cx1): the centre difference formed in LDS (centre staging), else read from the slots
+    HC_HD static Geo load_geo(const RF* base, const double* base64, long long nf, long long fidx, double cx0,
+                              double cx1)
+    {
+        Geo g;
+        if constexpr (MIXED) {
+            g.area = base64[FS_AREA * nf + fidx];
+            g.n0 = base64[FS_N0 * nf + fidx];
+            g.n1 = base64[FS_N1 * nf + fidx];
+        } else {
+            g.area = base[FS_AREA * nf + fidx];
+            g.n0 = base[FS_N0 * nf + fidx];
+            g.n1 = base[FS_N1 * nf + fidx];
+        }
+        g.dl = base[FS_DL * nf + fidx];
+        g.dr = base[FS_DR * nf + fidx];
+        g.s1 = base[FS_S1 * nf + fidx];
+        g.dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
+        g.dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
+        for (int d = 0; d < 3; ++d) {
+            g.fl[d] = base[(FS_FL0 + d) * nf + fidx];
+            g.fr[d] = base[(FS_FR0 + d) * nf + fidx];
+        }
+        return g;
+    }
+
     // MUSCL face flux (spatial/muscl.h:10-87) with precomputed face geometry
-    // (cx0, cx1): the centre difference dx_c formed in LDS (centre staging), else read from the slots
-    HC_HD static Flux4<FT> muscl(const StageArgs<RF>& a, const RF* base, const double* base64, long long nf,
-                                 long long fidx, const QT* ql, const QT* qr, const RF (*Ql)[2], const RF (*Qr)[2],
-                                 double cx0, double cx1)
+    HC_HD static Flux4<FT> muscl(const StageArgs<RF>& a, const Geo& g, const QT* ql, const QT* qr,
+                                 const RF (*Ql)[2], const RF (*Qr)[2])
     {
         if constexpr (MIXED) {
-            FaceG<double> f;
-            f.area = base64[FS_AREA * nf + fidx];
-            f.n0 = base64[FS_N0 * nf + fidx];
-            f.n1 = base64[FS_N1 * nf + fidx];
+            const FaceG<double> f{g.area, g.n0, g.n1};
             if (a.limiter == LIM_NOLIMIT1)
                 return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, ql, qr);
             const RF n0 = RF(f.n0), n1 = RF(f.n1);
-            const RF dl = base[FS_DL * nf + fidx];
-            const RF dr = base[FS_DR * nf + fidx];
-            const RF s1 = base[FS_S1 * nf + fidx];
-            const RF dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
-            const RF dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
-            const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
-            const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
             double qlf[NV], qrf[NV];
             for (int v = 0; v < NV; ++v) {
                 const RF dq = RF(qr[v] - ql[v]);
-                const RF dqdx = dq * s1;
-                const RF c0 = dq * dx0;
-                const RF c1 = dq * dx1;
-                const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
-                const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
+                const RF dqdx = dq * g.s1;
+                const RF c0 = dq * g.dx0;
+                const RF c1 = dq * g.dx1;
+                const RF gl = lsq_solve1(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
+                const RF gr = lsq_solve1(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
                 const RF slope_l = limit(a.limiter, dqdx, gl);
                 const RF slope_r = limit(a.limiter, dqdx, gr);
-                qlf[v] = ql[v] + double(slope_l * dl);
-                qrf[v] = qr[v] - double(slope_r * dr);
+                qlf[v] = ql[v] + double(slope_l * g.dl);
+                qrf[v] = qr[v] - double(slope_r * g.dr);
             }
             return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, qlf, qrf);
         } else {
-            FaceG<RF> f;
-            f.area = base[FS_AREA * nf + fidx];
-            f.n0 = base[FS_N0 * nf + fidx];
-            f.n1 = base[FS_N1 * nf + fidx];
+            const FaceG<RF> f{g.area, g.n0, g.n1};
             if (a.limiter == LIM_NOLIMIT1) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, ql, qr);
-            const RF dl = base[FS_DL * nf + fidx];
-            const RF dr = base[FS_DR * nf + fidx];
-            const RF s1 = base[FS_S1 * nf + fidx];
-            const RF dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
-            const RF dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
-            const RF fl[3] = {base[FS_FL0 * nf + fidx], base[FS_FL1 * nf + fidx], base[FS_FL2 * nf + fidx]};
-            const RF fr[3] = {base[FS_FR0 * nf + fidx], base[FS_FR1 * nf + fidx], base[FS_FR2 * nf + fidx]};
             RF qlf[NV], qrf[NV];
             for (int v = 0; v < NV; ++v) {
                 const RF dq = qr[v] - ql[v];
-                const RF dqdx = dq * s1;
-                const RF c0 = dq * dx0;
-                const RF c1 = dq * dx1;
-                const RF gl = lsq_solve1(fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
-                const RF gr = lsq_solve1(fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
+                const RF dqdx = dq * g.s1;
+                const RF c0 = dq * g.dx0;
+                const RF c1 = dq * g.dx1;
+                const RF gl = lsq_solve1(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
+                const RF gr = lsq_solve1(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
                 const RF slope_l = limit(a.limiter, dqdx, gl);
                 const RF slope_r = limit(a.limiter, dqdx, gr);
-                qlf[v] = ql[v] + slope_l * dl;
-                qrf[v] = qr[v] - slope_r * dr;
+                qlf[v] = ql[v] + slope_l * g.dl;
+                qrf[v] = qr[v] - slope_r * g.dr;
             }
             return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, qlf, qrf);
         }
@@ -413,7 +426,33 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
     // One face index space: i-faces k = li*TJ + lj (node line i0+li, row j0+lj), then j-faces
     // k = NFI + li*(TJ+1) + lj (row i0+li, node line j0+lj).  Both kinds run the same MUSCL and
     // flux code on lane-selected operands, so a wave holding both does not diverge.
-    HC_HD static void faces(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag)
+    // Geometry prefetch: the slots of the lane's first face (k = tid), loaded while the tile and
+    // the metric phase are in flight (their latency otherwise opens the face phase); the centre
+    // difference of centre staging is filled in the face phase
+    HC_HD static void prefetch(const StageArgs<RF>& a, int i0, int j0, int tid, Geo& g)
+    {
+        if constexpr (GPF != 0) {
+            const int k = tid;
+            if (k >= NF) return;
+            const bool isi = k < NFI;
+            const int kk = isi ? k : k - NFI;
+            const int w = isi ? TJ : TJ + 1;
+            const int li = kk / w, lj = kk % w;
+            const int l = isi ? i0 + li : j0 + lj;
+            const int t = isi ? j0 + lj : i0 + li;
+            const int lmax = isi ? a.m.ni : a.m.nj;
+            if (l > lmax || t >= (isi ? a.m.nj : a.m.ni)) return;
+            const bool interior = l >= 1 && l <= lmax - 1;
+            if (!(interior || (isi ? a.m.per_i : a.m.per_j))) return;
+            const int le = interior ? l : 0;
+            const int nj = a.m.nj;
+            const long long fidx = isi ? (long long)le * nj + t : (long long)t * (nj + 1) + le;
+            g = load_geo(isi ? a.m.fi : a.m.fj, isi ? a.m.fi64 : a.m.fj64, isi ? a.m.nfi : a.m.nfj, fidx, 0.0, 0.0);
+        }
+    }
+
+    HC_HD static void faces(const StageArgs<RF>& a, Smem& s, int i0, int j0, int tid, unsigned int& flag,
+                            const Geo& pre)
     {
         const int ni = a.m.ni, nj = a.m.nj;
         const QT* qs = &s.q[0][0][0];
@@ -450,7 +489,16 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI 
                 }
                 double cx0 = 0.0, cx1 = 0.0;
                 if constexpr (CSTAGE) centre_diff(a, s, pl, pr, !interior, isi, cx0, cx1);
-                store(s, k, muscl(a, base, base64, nf, fidx, ql, qr, Ql, Qr, cx0, cx1));
+                if (GPF != 0 && k == tid) {
+                    Geo g = pre;
+                    if constexpr (CSTAGE) {
+                        g.dx0 = RF(cx0);
+                        g.dx1 = RF(cx1);
+                    }
+                    store(s, k, muscl(a, g, ql, qr, Ql, Qr));
+                } else {
+                    store(s, k, muscl(a, load_geo(base, base64, nf, fidx, cx0, cx1), ql, qr, Ql, Qr));
+                }
             } else {
                 // the interior cell is right of node line 0 and left of the last node line
                 const bool first = l == 0;
@@ -699,10 +747,10 @@ __device__ inline unsigned int wave_or(unsigned int v)
 #define HC_STAGE_OCCUPANCY
 #endif
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST>
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF>
 __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF> L)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST, GPF>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
@@ -715,10 +763,12 @@ __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF
     const int i0 = (lt / c.tiles_j) * TI, j0 = (lt % c.tiles_j) * TJ;
     unsigned int flag = 0;
     T::load(a, s, i0, j0, tid);
+    typename T::Geo g0;
+    T::prefetch(a, i0, j0, tid, g0);
     __syncthreads();
     T::qmetric(a, s, i0, j0, tid);
     __syncthreads();
-    T::faces(a, s, i0, j0, tid, flag);
+    T::faces(a, s, i0, j0, tid, flag, g0);
     __syncthreads();
     double red;
     T::cells(a, s, i0, j0, tid, flag, red);

@@ -79,9 +79,10 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
     return a;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST> void run_stage(const StageArgs<RF>& a)
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST, int GPF>
+void run_stage(const StageArgs<RF>& a)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST, GPF>;
     const int tiles_i = (a.m.ni + TI - 1) / TI, tiles_j = (a.m.nj + TJ - 1) / TJ;
     const int ntiles = tiles_i * tiles_j;
     auto sp = std::make_unique<typename T::Smem>();
@@ -91,9 +92,13 @@ template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CS
         const int t = xcd_tile(b, ntiles);
         const int i0 = (t / tiles_j) * TI, j0 = (t % tiles_j) * TJ;
         for (auto& f : flag) f = 0;
-        for (int tid = 0; tid < T::NT; ++tid) T::load(a, s, i0, j0, tid);
+        std::vector<typename T::Geo> g0(T::NT);
+        for (int tid = 0; tid < T::NT; ++tid) {
+            T::load(a, s, i0, j0, tid);
+            T::prefetch(a, i0, j0, tid, g0[tid]);
+        }
         for (int tid = 0; tid < T::NT; ++tid) T::qmetric(a, s, i0, j0, tid);
-        for (int tid = 0; tid < T::NT; ++tid) T::faces(a, s, i0, j0, tid, flag[tid]);
+        for (int tid = 0; tid < T::NT; ++tid) T::faces(a, s, i0, j0, tid, flag[tid], g0[tid]);
         std::vector<double> red(T::NT);
         for (int tid = 0; tid < T::NT; ++tid) T::cells(a, s, i0, j0, tid, flag[tid], red[tid]);
         double m = 0.0;
@@ -115,24 +120,24 @@ template <int LAW, int BASIS, class RF> void run_update(const StageArgs<RF>& a)
     *a.status |= flag;
 }
 
-template <class RF, int TI, int TJ, int NT, int CST> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
+template <class RF, int TI, int TJ, int NT, int CST, int GPF> void dispatch_stage(const hc_case_desc& cs, const StageArgs<RF>& a)
 {
     const int key = cs.law * 100 + cs.basis * 10 + cs.flux;
     switch (key) {
-    case 0: run_stage<0, 0, 0, RF, TI, TJ, NT, CST>(a); break;
-    case 1: run_stage<0, 0, 1, RF, TI, TJ, NT, CST>(a); break;
-    case 100: run_stage<1, 0, 0, RF, TI, TJ, NT, CST>(a); break;
-    case 101: run_stage<1, 0, 1, RF, TI, TJ, NT, CST>(a); break;
-    case 102: run_stage<1, 0, 2, RF, TI, TJ, NT, CST>(a); break;
-    case 103: run_stage<1, 0, 3, RF, TI, TJ, NT, CST>(a); break;
-    case 104: run_stage<1, 0, 4, RF, TI, TJ, NT, CST>(a); break;
-    case 105: run_stage<1, 0, 5, RF, TI, TJ, NT, CST>(a); break;
-    case 110: run_stage<1, 1, 0, RF, TI, TJ, NT, CST>(a); break;
-    case 111: run_stage<1, 1, 1, RF, TI, TJ, NT, CST>(a); break;
-    case 112: run_stage<1, 1, 2, RF, TI, TJ, NT, CST>(a); break;
-    case 113: run_stage<1, 1, 3, RF, TI, TJ, NT, CST>(a); break;
-    case 114: run_stage<1, 1, 4, RF, TI, TJ, NT, CST>(a); break;
-    case 115: run_stage<1, 1, 5, RF, TI, TJ, NT, CST>(a); break;
+    case 0: run_stage<0, 0, 0, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 1: run_stage<0, 0, 1, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 100: run_stage<1, 0, 0, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 101: run_stage<1, 0, 1, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 102: run_stage<1, 0, 2, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 103: run_stage<1, 0, 3, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 104: run_stage<1, 0, 4, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 105: run_stage<1, 0, 5, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 110: run_stage<1, 1, 0, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 111: run_stage<1, 1, 1, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 112: run_stage<1, 1, 2, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 113: run_stage<1, 1, 3, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 114: run_stage<1, 1, 4, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 115: run_stage<1, 1, 5, RF, TI, TJ, NT, CST, GPF>(a); break;
     default: break;
     }
 }
@@ -151,9 +156,10 @@ int g_tile_cfg = 0;
 template <int K> void stage_shape(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
     constexpr TileShape S64 = TILES64[K], S32 = TILES32[K];
-    if (e.precision != HC_PRECISION_MIXED) dispatch_stage<double, S64.ti, S64.tj, S64.nt, S64.cst>(e.cs, a64);
-    else if (mixed_wide(e.cs.law, e.cs.flux)) dispatch_stage<float, S64.ti, S64.tj, S64.nt, S64.cst>(e.cs, a32);
-    else dispatch_stage<float, S32.ti, S32.tj, S32.nt, S32.cst>(e.cs, a32);
+    if (e.precision != HC_PRECISION_MIXED) dispatch_stage<double, S64.ti, S64.tj, S64.nt, S64.cst, S64.gpf>(e.cs, a64);
+    else if (mixed_wide(e.cs.law, e.cs.flux))
+        dispatch_stage<float, S64.ti, S64.tj, S64.nt, S64.cst, S64.gpf>(e.cs, a32);
+    else dispatch_stage<float, S32.ti, S32.tj, S32.nt, S32.cst, S32.gpf>(e.cs, a32);
 }
 
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
