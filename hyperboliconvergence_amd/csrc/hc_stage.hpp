@@ -516,8 +516,41 @@ struct StageTile {
 
     // ---- phase 4: owner-computes residual accumulation and the stage update ----
     // red: the cell's lambda/vol for the global-dt reduction of stage 0 (0 otherwise)
+    // Cell prefetch (FP64 kernels with the geometry prefetch, which have the registers to spare at
+    // 4 waves per SIMD): the cell phase's inputs of the lane's cell (step state, the accumulator
+    // read first, volume, stage-0 spectral radius) are loaded into registers when the face phase
+    // opens, so the cell phase does not start with a round trip to memory
+    static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double);
+    struct CellIn {
+        double q0[NV], acc[NV], vol, lam0;
+    };
+
+    // the accumulator value the cell phase reads first: the stage's base slot, else the old value
+    // of its first update slot (-1: none)
+    HC_HD static int first_acc_slot(const RkPlan& p)
+    {
+        if (p.base >= 0) return p.base;
+        if (p.nupd > 0 && !p.upd_first[0]) return p.upd_slot[0];
+        return -1;
+    }
+
+    HC_HD static void prefetch_cell(const StageArgs<RF>& a, int i0, int j0, int tid, CellIn& ci)
+    {
+        if constexpr (CPF) {
+            const int i = i0 + tid / TJ, j = j0 + tid % TJ;
+            if (tid >= NC || i >= a.m.ni || j >= a.m.nj || a.mode == MODE_RESIDUAL) return;
+            const long long c = (long long)i * a.m.nj + j;
+            for (int v = 0; v < NV; ++v) ci.q0[v] = a.q0[v * a.N + c];
+            const int fs = first_acc_slot(a.plan);
+            if (fs >= 0)
+                for (int v = 0; v < NV; ++v) ci.acc[v] = a.acc[((long long)fs * NV + v) * a.N + c];
+            ci.vol = a.m.cvol[c];
+            if (a.dt_mode == 1 && a.stage != 0) ci.lam0 = a.lam0[c];
+        }
+    }
+
     HC_HD static void cells(const StageArgs<RF>& a, const Smem& s, int i0, int j0, int tid, unsigned int& flag,
-                            double& red)
+                            double& red, const CellIn& ci)
     {
         const int ni = a.m.ni, nj = a.m.nj;
         const int li = tid / TJ, lj = tid % TJ;
@@ -567,7 +600,8 @@ struct StageTile {
         const RkPlan& p = a.plan;
         double Rt[NV];
         for (int v = 0; v < NV; ++v) {
-            const double base = p.base >= 0 ? a.acc[((long long)p.base * NV + v) * a.N + c] : 0.0;
+            double base = 0.0;
+            if (p.base >= 0) base = CPF ? ci.acc[v] : a.acc[((long long)p.base * NV + v) * a.N + c];
             Rt[v] = base + R[v] * p.a_ss;
         }
         // unrolled over the plan's fixed capacity: constant indices only (the arguments may live in registers)
@@ -576,11 +610,12 @@ struct StageTile {
             if (u >= p.nupd) break;
             for (int v = 0; v < NV; ++v) {
                 double* A = &a.acc[((long long)p.upd_slot[u] * NV + v) * a.N + c];
-                const double old = p.upd_first[u] ? 0.0 : *A;
+                double old = 0.0;
+                if (!p.upd_first[u]) old = (CPF && u == 0 && p.base < 0) ? ci.acc[v] : *A;
                 *A = old + R[v] * p.upd_coef[u];
             }
         }
-        const double vol = a.m.cvol[c];
+        const double vol = CPF ? ci.vol : a.m.cvol[c];
         if (a.mode == MODE_GLOBAL_STAGE0) {
             for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
             red = lam / vol;   // spectralRadius.h:399-401
@@ -598,18 +633,20 @@ struct StageTile {
                 l0 = lam;
                 a.lam0[c] = lam;
             } else {
-                l0 = a.lam0[c];
+                l0 = CPF ? ci.lam0 : a.lam0[c];
             }
             fac = (p.beta * a.cfl) / l0;
         }
-        update_cell(a, c, Rt, fac, flag);
+        double q0[NV];
+        for (int v = 0; v < NV; ++v) q0[v] = CPF ? ci.q0[v] : a.q0[v * a.N + c];
+        update_cell(a, c, q0, Rt, fac, flag);
     }
 
     // q_out = S( C(q0) + R_tot*fac )   (spatial/eulerForwardUpdate.h:274-283)
-    HC_HD static void update_cell(const StageArgs<RF>& a, long long c, const double* Rt, double fac, unsigned int& flag)
+    HC_HD static void update_cell(const StageArgs<RF>& a, long long c, const double* q0, const double* Rt, double fac,
+                                  unsigned int& flag)
     {
-        double q0[NV], qc[NV], qn[NV];
-        for (int v = 0; v < NV; ++v) q0[v] = a.q0[v * a.N + c];
+        double qc[NV], qn[NV];
         to_conserved<double, LAW, BASIS>(a.sp, q0, qc);
         for (int v = 0; v < NV; ++v) qc[v] = qc[v] + Rt[v] * fac;
         from_conserved<double, LAW, BASIS>(a.sp, qc, qn);
@@ -642,9 +679,12 @@ template <int LAW, int BASIS> struct Stage0Update {
         if (c >= a.N) return;
         const double stepfac = a.plan.beta * dt;
         const double fac = stepfac / a.m.cvol[c];
-        double Rt[NV];
-        for (int v = 0; v < NV; ++v) Rt[v] = a.tmp[v * a.N + c];
-        StageTile<LAW, BASIS, FLUX_CENTRAL, RF, 1, 64>::update_cell(a, c, Rt, fac, flag);
+        double Rt[NV], q0[NV];
+        for (int v = 0; v < NV; ++v) {
+            Rt[v] = a.tmp[v * a.N + c];
+            q0[v] = a.q0[v * a.N + c];
+        }
+        StageTile<LAW, BASIS, FLUX_CENTRAL, RF, 1, 64>::update_cell(a, c, q0, Rt, fac, flag);
     }
 };
 
@@ -768,10 +808,12 @@ __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF
     __syncthreads();
     T::qmetric(a, s, i0, j0, tid);
     __syncthreads();
+    typename T::CellIn cin;
+    T::prefetch_cell(a, i0, j0, tid, cin);
     T::faces(a, s, i0, j0, tid, flag, g0);
     __syncthreads();
     double red;
-    T::cells(a, s, i0, j0, tid, flag, red);
+    T::cells(a, s, i0, j0, tid, flag, red, cin);
     // work-group reductions: max(lambda/vol) for global-dt stage 0 and the status bits
     const double m = wave_max(red);
     const unsigned int f = wave_or(flag);

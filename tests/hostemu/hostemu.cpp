@@ -100,7 +100,9 @@ void run_stage(const StageArgs<RF>& a)
         for (int tid = 0; tid < T::NT; ++tid) T::qmetric(a, s, i0, j0, tid);
         for (int tid = 0; tid < T::NT; ++tid) T::faces(a, s, i0, j0, tid, flag[tid], g0[tid]);
         std::vector<double> red(T::NT);
-        for (int tid = 0; tid < T::NT; ++tid) T::cells(a, s, i0, j0, tid, flag[tid], red[tid]);
+        std::vector<typename T::CellIn> ci(T::NT);
+        for (int tid = 0; tid < T::NT; ++tid) T::prefetch_cell(a, i0, j0, tid, ci[tid]);
+        for (int tid = 0; tid < T::NT; ++tid) T::cells(a, s, i0, j0, tid, flag[tid], red[tid], ci[tid]);
         double m = 0.0;
         unsigned int f = 0;
         for (int tid = 0; tid < T::NT; ++tid) {
