@@ -323,9 +323,10 @@ int build_layout(hc_context* ctx)
                          hipMemcpyHostToDevice));
         HC_HIP(hipMemcpy(ctx->fj + NSLOT * c.fj_off, c.g.fj.data(), c.g.fj.size() * sizeof(double),
                          hipMemcpyHostToDevice));
-        std::vector<float> f32(c.g.fi.begin(), c.g.fi.end());
+        std::vector<float> f32;
+        fp32_slots(c.g.fi, c.nfi, f32);
         HC_HIP(hipMemcpy(ctx->fi32 + NSLOT * c.fi_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
-        f32.assign(c.g.fj.begin(), c.g.fj.end());
+        fp32_slots(c.g.fj, c.nfj, f32);
         HC_HIP(hipMemcpy(ctx->fj32 + NSLOT * c.fj_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
         c.g = GeomOutput{};   // the device holds it now
     }

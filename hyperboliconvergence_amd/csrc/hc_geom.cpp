@@ -215,4 +215,19 @@ int build_geometry(const GeomInput& in, GeomOutput& out)
     return 0;
 }
 
+void fp32_slots(const std::vector<double>& f64, long long nf, std::vector<float>& f32)
+{
+    f32.assign(f64.begin(), f64.end());
+    for (long long k = 0; k < nf; ++k) {
+        const double n0 = f64[FS_N0 * nf + k], n1 = f64[FS_N1 * nf + k];
+        for (int side = 0; side < 2; ++side) {
+            const int s0 = side == 0 ? FS_FL0 : FS_FR0;
+            const double x0 = f64[s0 * nf + k], x1 = f64[(s0 + 1) * nf + k], x2 = f64[(s0 + 2) * nf + k];
+            f32[s0 * nf + k] = float(x0 * n0 + x1 * n1);
+            f32[(s0 + 1) * nf + k] = float(x1 * n0 + x2 * n1);
+            f32[(s0 + 2) * nf + k] = 0.0f;
+        }
+    }
+}
+
 }   // namespace hc

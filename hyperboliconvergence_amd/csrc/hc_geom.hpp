@@ -29,4 +29,9 @@ struct GeomOutput {
 // returns 0 on success, -1 on a degenerate cell or face
 int build_geometry(const GeomInput& in, GeomOutput& out);
 
+// FP32 face slots of the mixed path from the FP64 ones: every slot rounded, except that the two
+// biased LSQ factors become their directional weights w = X'^-1 n (two per side: the slope along
+// n is Q'.w), formed in FP64 and rounded
+void fp32_slots(const std::vector<double>& f64, long long nf, std::vector<float>& f32);
+
 }   // namespace hc

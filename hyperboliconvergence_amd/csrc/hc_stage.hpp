@@ -39,6 +39,9 @@ enum FaceSlot {
 // boundary (non-periodic) faces store the face centre in FS_DL / FS_DR
 constexpr int FS_CX = FS_DL;
 constexpr int FS_CY = FS_DR;
+// the FP32 slots of the mixed path hold the directional weights w = X'^-1 n of the biased LSQ
+// factors in place of the factors (two per side)
+constexpr int FS_WL0 = FS_FL0, FS_WL1 = FS_FL1, FS_WR0 = FS_FR0, FS_WR1 = FS_FR1;
 
 // status bits returned through the C-ABI
 enum StatusBits : unsigned int {
@@ -326,11 +329,26 @@ struct StageTile {
         g.s1 = base[FS_S1 * nf + fidx];
         g.dx0 = CSTAGE ? RF(cx0) : base[FS_DX0 * nf + fidx];
         g.dx1 = CSTAGE ? RF(cx1) : base[FS_DX1 * nf + fidx];
-        for (int d = 0; d < 3; ++d) {
-            g.fl[d] = base[(FS_FL0 + d) * nf + fidx];
-            g.fr[d] = base[(FS_FR0 + d) * nf + fidx];
+        if constexpr (sizeof(RF) == sizeof(float)) {
+            g.fl[0] = base[FS_WL0 * nf + fidx];
+            g.fl[1] = base[FS_WL1 * nf + fidx];
+            g.fr[0] = base[FS_WR0 * nf + fidx];
+            g.fr[1] = base[FS_WR1 * nf + fidx];
+        } else {
+            for (int d = 0; d < 3; ++d) {
+                g.fl[d] = base[(FS_FL0 + d) * nf + fidx];
+                g.fr[d] = base[(FS_FR0 + d) * nf + fidx];
+            }
         }
         return g;
+    }
+
+    // gradient along n from the biased LSQ metric: the factor solve on FP64 slots
+    // (lsq/factor_solve.ipp:443-459), the directional weights on the FP32 slots
+    HC_HD static RF directional(const RF* f, RF q0, RF q1, RF n0, RF n1)
+    {
+        if constexpr (sizeof(RF) == sizeof(float)) return q0 * f[0] + q1 * f[1];
+        else return lsq_solve1(f, q0, q1, n0, n1);
     }
 
     // MUSCL face flux (spatial/muscl.h:10-87) with precomputed face geometry
@@ -348,8 +366,8 @@ struct StageTile {
                 const RF dqdx = dq * g.s1;
                 const RF c0 = dq * g.dx0;
                 const RF c1 = dq * g.dx1;
-                const RF gl = lsq_solve1(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
-                const RF gr = lsq_solve1(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
+                const RF gl = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
+                const RF gr = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
                 const RF slope_l = limit(a.limiter, dqdx, gl);
                 const RF slope_r = limit(a.limiter, dqdx, gr);
                 qlf[v] = ql[v] + double(slope_l * g.dl);
@@ -365,8 +383,8 @@ struct StageTile {
                 const RF dqdx = dq * g.s1;
                 const RF c0 = dq * g.dx0;
                 const RF c1 = dq * g.dx1;
-                const RF gl = lsq_solve1(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
-                const RF gr = lsq_solve1(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
+                const RF gl = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
+                const RF gr = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
                 const RF slope_l = limit(a.limiter, dqdx, gl);
                 const RF slope_r = limit(a.limiter, dqdx, gr);
                 qlf[v] = ql[v] + slope_l * g.dl;

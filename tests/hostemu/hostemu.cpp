@@ -189,8 +189,8 @@ int setup(Emu& e, const hc_case_desc* cs, int ni, int nj, const double* nodes, c
     GeomInput gin{ni, nj, nodes, {cs->bc[0], cs->bc[1], cs->bc[2], cs->bc[3]}, {cs->period_i[0], cs->period_i[1]},
                   {cs->period_j[0], cs->period_j[1]}};
     if (build_geometry(gin, e.g) != 0) return -1;
-    e.fi32.assign(e.g.fi.begin(), e.g.fi.end());
-    e.fj32.assign(e.g.fj.begin(), e.g.fj.end());
+    fp32_slots(e.g.fi, (long long)(ni + 1) * nj, e.fi32);
+    fp32_slots(e.g.fj, (long long)ni * (nj + 1), e.fj32);
     const long long N = (long long)ni * nj;
     for (auto& b : e.q) b.assign((size_t)e.nv * N, 0.0);
     for (long long c = 0; c < N; ++c)
