@@ -805,8 +805,12 @@ __device__ inline unsigned int wave_or(unsigned int v)
 #define HC_STAGE_OCCUPANCY
 #endif
 
+// The prefetching configurations hold the prefetched values across the metric phase: at least 4
+// waves per SIMD keeps them within 128 VGPRs (the AUSM+up FP64 kernel would otherwise take 129
+// and drop to 3 waves)
 template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF>
-__global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY k_stage(LaunchArgs<RF> L)
+__global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GPF != 0 ? 4 : 1)))
+k_stage(LaunchArgs<RF> L)
 {
     using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST, GPF>;
     __shared__ typename T::Smem s;
