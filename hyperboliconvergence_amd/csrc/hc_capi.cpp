@@ -17,10 +17,11 @@ namespace {
 
 using namespace hc;
 
+// n: slots (MAXSPREAD per case)
 __global__ void k_reset_max(unsigned long long* p, int n)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) p[k] = dbl_bits(DBL_MIN);
+    if (k < n) p[(long long)k * MAXSTRIDE] = dbl_bits(DBL_MIN);
 }
 
 __global__ void k_clear_dt(double* p, int n)
@@ -342,10 +343,10 @@ int build_layout(hc_context* ctx)
     HC_HIP(hipMalloc(&ctx->res, (size_t)(nv + 1) * N * sizeof(double)));
     HC_HIP(hipMalloc(&ctx->bref, (size_t)br * nv * sizeof(double)));
     HC_HIP(hipMemset(ctx->bref, 0, (size_t)br * nv * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->maxslot, nc * sizeof(unsigned long long)));
+    HC_HIP(hipMalloc(&ctx->maxslot, (size_t)nc * MAXSPREAD * MAXSTRIDE * sizeof(unsigned long long)));
     HC_HIP(hipMalloc(&ctx->dtslot, 2 * nc * sizeof(double)));
     {
-        std::vector<unsigned long long> m(nc, dbl_bits(DBL_MIN));
+        std::vector<unsigned long long> m((size_t)nc * MAXSPREAD * MAXSTRIDE, dbl_bits(DBL_MIN));
         std::vector<double> z(2 * nc, 0.0);
         HC_HIP(hipMemcpy(ctx->maxslot, m.data(), m.size() * sizeof(m[0]), hipMemcpyHostToDevice));
         HC_HIP(hipMemcpy(ctx->dtslot, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -538,7 +539,8 @@ int enqueue_step(hc_context* ctx)
     }
     if (global && ns == 1) {
         const int nc = (int)ctx->cases.size();
-        hipLaunchKernelGGL(k_reset_max, dim3((nc + 255) / 256), dim3(256), 0, ctx->stream, ctx->maxslot, nc);
+        const int nslot = nc * MAXSPREAD;
+        hipLaunchKernelGGL(k_reset_max, dim3((nslot + 255) / 256), dim3(256), 0, ctx->stream, ctx->maxslot, nslot);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hipfail(ctx, e, "reset kernel launch");
     }

@@ -61,7 +61,7 @@ struct hc_context {
     int cur = 0;
     double *acc = nullptr, *tmp = nullptr, *lam0 = nullptr, *bref = nullptr, *res = nullptr, *part = nullptr,
            *nout = nullptr;
-    unsigned long long* maxslot = nullptr;   // one per case
+    unsigned long long* maxslot = nullptr;   // MAXSPREAD slots per case, MAXSTRIDE words apart
     double* dtslot = nullptr;                // two per case
     unsigned int* status = nullptr;
     CaseRec<double>* rec64[NTILECFG] = {};

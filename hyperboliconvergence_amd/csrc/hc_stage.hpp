@@ -79,6 +79,12 @@ struct RkPlan {
 
 enum StageMode { MODE_FUSED = 0, MODE_GLOBAL_STAGE0 = 1, MODE_RESIDUAL = 2 };
 
+// A case's max(lambda/vol) of global-dt stage 0 is reduced into MAXSPREAD slots, each on its own
+// 128-byte line (MAXSTRIDE words apart), one atomic per work-group on the slot of its tile index:
+// atomics on one address serialise (a 4096^2 grid has 524288 waves), atomics on 64 lines do not.
+// The update kernel takes the max of the slots with one load per lane of a wavefront.
+constexpr int MAXSPREAD = 64, MAXSTRIDE = 16;
+
 // Accumulation plans of a Runge-Kutta table in the reference's (alpha, beta) form
 // (ode.h:87-105): one accumulator per later row with a non-zero coefficient
 // before its diagonal.  Returns the number of accumulators.
@@ -128,7 +134,7 @@ template <class RF> struct StageArgs {
     double* lam0;         // stage-0 spectral radius sums [N]
     const double* bref;   // boundary reference values, AoS [2nj+2ni][nv]
     double* res_out;      // residual-only mode: AoS [N][nv+1]
-    unsigned long long* maxslot;   // bits of max(lambda/vol), reset to bits(DBL_MIN)
+    unsigned long long* maxslot;   // bits of max(lambda/vol), MAXSPREAD slots MAXSTRIDE apart, reset to bits(DBL_MIN)
     double* dtslot;       // [0] dt of the current step, [1] accumulated time
     unsigned int* status;
     RkPlan plan;
@@ -685,10 +691,10 @@ struct StageTile {
 // deferred update of global-dt stage 0 (dt = cfl / max(lambda/vol), rungeKutta.h:133)
 template <int LAW, int BASIS> struct Stage0Update {
     static constexpr int NV = LawTraits<LAW>::nv;
+    // mx: the case's max(lambda/vol) over its slots
     template <class RF>
-    HC_HD static void run(const StageArgs<RF>& a, long long c, unsigned int& flag)
+    HC_HD static void run(const StageArgs<RF>& a, long long c, unsigned int& flag, double mx)
     {
-        const double mx = bits_dbl(*a.maxslot);
         const double dt = a.cfl / mx;
         if (c == 0) {
             a.dtslot[0] = dt;
@@ -741,7 +747,7 @@ template <class RF> struct LaunchArgs {
     double* lam0;
     const double* bref;   // packed [rows_c][nv] blocks
     double* res_out;      // packed [N_c][nv+1] blocks
-    unsigned long long* maxslot;   // one per case
+    unsigned long long* maxslot;   // MAXSPREAD * MAXSTRIDE per case
     double* dtslot;                // two per case
     unsigned int* status;
 };
@@ -771,7 +777,7 @@ template <class RF, int NV> HC_HD StageArgs<RF> case_args(const LaunchArgs<RF>& 
     a.lam0 = L.lam0 + c.cell_off;
     a.bref = L.bref + NV * c.bref_off;
     a.res_out = L.res_out + (NV + 1) * c.cell_off;
-    a.maxslot = L.maxslot + ci;
+    a.maxslot = L.maxslot + (long long)ci * MAXSPREAD * MAXSTRIDE;
     a.dtslot = L.dtslot + 2 * ci;
     a.status = L.status;
     a.plan = L.plan;
@@ -797,6 +803,15 @@ __device__ inline unsigned int wave_or(unsigned int v)
     for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);
     return v;
 }
+__device__ inline unsigned long long wave_max_u64(unsigned long long v)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+static_assert(MAXSPREAD == 64, "one slot per lane of a wavefront");
 
 // optional occupancy hint (waves per SIMD) for the register allocator, set per build
 #if defined(HC_STAGE_WAVES_PER_EU) && HC_STAGE_WAVES_PER_EU > 0
@@ -836,15 +851,34 @@ k_stage(LaunchArgs<RF> L)
     __syncthreads();
     double red;
     T::cells(a, s, i0, j0, tid, flag, red, cin);
-    // work-group reductions: max(lambda/vol) for global-dt stage 0 and the status bits
+    // reductions: the status bits; for global-dt stage 0 also max(lambda/vol), over the
+    // work-group and into the tile's slot (mode is uniform over the grid: the barrier is too)
     const double m = wave_max(red);
     const unsigned int f = wave_or(flag);
-    if ((tid & 63) == 0) {
-        if (a.mode == MODE_GLOBAL_STAGE0) atomicMax(a.maxslot, dbl_bits(m));
-        if (f) atomicOr(a.status, f);
+    if (a.mode == MODE_GLOBAL_STAGE0) {
+        static_assert(T::NT % 64 == 0, "whole wavefronts");
+        __shared__ unsigned long long wm[T::NT / 64];
+        __shared__ unsigned int wf[T::NT / 64];
+        if ((tid & 63) == 0) {
+            wm[tid >> 6] = dbl_bits(m);
+            wf[tid >> 6] = f;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            unsigned long long mm = wm[0];
+            unsigned int ff = wf[0];
+            for (int w = 1; w < T::NT / 64; ++w) {
+                mm = wm[w] > mm ? wm[w] : mm;
+                ff |= wf[w];
+            }
+            atomicMax(a.maxslot + (lt % MAXSPREAD) * MAXSTRIDE, mm);
+            if (ff) atomicOr(a.status, ff);
+        }
+    } else if ((tid & 63) == 0 && f) {
+        atomicOr(a.status, f);
     }
-    // the last stage re-arms the case's max slot for the next step (no stage reads it then)
-    if (a.reset_max && lt == 0 && tid == 0) *a.maxslot = dbl_bits(DBL_MIN);
+    // the last stage re-arms the case's max slots for the next step (no stage reads them then)
+    if (a.reset_max && lt == 0 && tid < MAXSPREAD) a.maxslot[tid * MAXSTRIDE] = dbl_bits(DBL_MIN);
     // fixed step: the case's time advances once per step (one thread of the case's first tile)
     if (a.dt_mode == 2 && a.mode == MODE_FUSED && a.stage == 0 && lt == 0 && tid == 0) {
         a.dtslot[0] = a.cfl;
@@ -858,8 +892,10 @@ __global__ void __launch_bounds__(256) k_stage0_update(LaunchArgs<RF> L, const U
     const UpdChunk ch = chunks[blockIdx.x];
     const StageArgs<RF> a = case_args<RF, LawTraits<LAW>::nv>(L, L.cases[ch.ci], ch.ci);
     const long long c = (long long)ch.start + threadIdx.x;
+    // every lane is active here: one slot per lane, then the wavefront's max
+    const double mx = bits_dbl(wave_max_u64(a.maxslot[(threadIdx.x & 63) * MAXSTRIDE]));
     unsigned int flag = 0;
-    Stage0Update<LAW, BASIS>::run(a, c, flag);
+    Stage0Update<LAW, BASIS>::run(a, c, flag, mx);
     const unsigned int f = wave_or(flag);
     if ((threadIdx.x & 63) == 0 && f) atomicOr(a.status, f);
 }

@@ -118,7 +118,9 @@ void run_stage(const StageArgs<RF>& a)
 template <int LAW, int BASIS, class RF> void run_update(const StageArgs<RF>& a)
 {
     unsigned int flag = 0;
-    for (long long c = 0; c < a.N; ++c) Stage0Update<LAW, BASIS>::run(a, c, flag);
+    // the device spreads the max over slots; its max over them is this one slot's value
+    const double mx = bits_dbl(*a.maxslot);
+    for (long long c = 0; c < a.N; ++c) Stage0Update<LAW, BASIS>::run(a, c, flag, mx);
     *a.status |= flag;
 }
 
