@@ -11,7 +11,9 @@ exercise: indexing, tiling, batching and graph replay at their real shapes).
   after 20 steps of the vortex);
 * every tile configuration gives the same bits on the 512 x 256 cylinder and the 96-case batch;
 * graph replay equals eager stepping on the cylinder;
-* the mixed path stays within the north-star bound of FP64 on the 96-case batch.
+* the mixed path stays within the north-star bound of FP64 on the 96-case batch;
+* on the 4096^2 vortex the mixed path departs from FP64 no further than FP64 does from itself
+  when its initial state is perturbed by an FP32-sized amount.
 """
 import numpy as np
 import pytest
@@ -64,6 +66,36 @@ def test_vortex_4096_conserves_mass_momentum_energy(precision):
     err = _conservation_error(cases.prim_to_cons(cfg["q"]), cases.prim_to_cons(q))
     print("vortex 4096^2, 20 steps, conservation error", err)
     assert err <= CONSERVATION_TOL
+
+
+def test_vortex_4096_mixed_error_is_the_cases_sensitivity_to_fp32_sized_perturbations():
+    """On the 4096^2 vortex the mixed path ends 0.1 away from FP64 after 500 steps, while it stays
+    within 1e-6 at 1024^2.  The case, not the mixed arithmetic, sets that: FP64 itself, started
+    from the initial state perturbed by 1e-7 (an FP32 rounding), ends as far away, whereas a 1e-9
+    perturbation stays in the linear regime (it grows to ~1e-9).  The flux has discrete switches
+    (|M| > 1 selects P1, which the reference makes identically 1, so the pressure flux jumps from
+    about p to about 2p at |M| = 1); perturbations of FP32 size flip some of them in the
+    supersonic ring of this grid, smaller ones do not."""
+    import hyperboliconvergence_amd as hca
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_c3(4096, hca.FLUX_AUSMPUP)
+    sign = np.random.default_rng(11).choice([-1.0, 1.0], size=cfg["q"].shape)
+
+    def run(q, precision=0):
+        with _ctx(cfg, precision, graph_steps=50) as ctx:
+            ctx.set_state(q)
+            ctx.step(500)
+            assert ctx.status() == 0
+            return ctx.get_state()
+
+    ref = run(cfg["q"])
+    e_mixed = rel_err(run(cfg["q"], 1), ref)
+    e_9 = rel_err(run(cfg["q"] * (1.0 + 1e-9 * sign)), ref)
+    e_7 = rel_err(run(cfg["q"] * (1.0 + 1e-7 * sign)), ref)
+    print(f"vortex 4096^2, 500 steps: mixed vs FP64 {e_mixed:.3e}; FP64 perturbed by 1e-7: {e_7:.3e}, by 1e-9: {e_9:.3e}")
+    assert e_9 <= 1e-8
+    assert e_7 >= 1e-3
+    assert e_mixed <= 10.0 * e_7
 
 
 def test_scalar_128_conserves_the_scalar():
