@@ -131,6 +131,31 @@ def test_vortex_trajectory_matches_oracle(flux):
     assert abs(t - to) <= 1e-12 * abs(to)
 
 
+@pytest.mark.parametrize("rk", ["ssp11", "ssp22", "ssp33", "ssp34", "rk4"])
+def test_global_time_step_every_rk_table_matches_oracle(rk):
+    """Global dt on a grid of more tiles than the per-case max slots (128^2: 128 tiles of 4 x 32),
+    eager and in graphs; ssp11 re-arms the slots with its own kernel after each step."""
+    from hyperboliconvergence_amd import cases
+    cfg = _vortex_cfg(128, 4)
+    n, a, b = cases.rk_table(rk)
+    cs = cfg["case"]
+    cs.nstages = n
+    for s in range(6):
+        cs.beta[s] = b[s]
+        for k in range(6):
+            cs.alpha[s][k] = a[s, k]
+    qo, to, _ = po.integrate(cs, cfg["nodes"], cfg["q"], 6, cfg["bref"])
+    for graph_steps in (0, 3):
+        with _ctx(cfg) as ctx:
+            ctx.set_tiles(0)
+            ctx.set_graph_steps(graph_steps)
+            ctx.step(6)
+            q = ctx.get_state()
+            t = ctx.time()
+        assert rel_err(q, qo) <= TRAJ_TOL, graph_steps
+        assert abs(t - to) <= 1e-12 * abs(to), graph_steps
+
+
 def test_graph_replay_matches_eager():
     cfg = _cyl_cfg(64, 32)
     with _ctx(cfg) as a:
