@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import (BASIS_CONSERVED, BASIS_PRIMITIVE, BC_PERIODIC, BC_RIEMANN, BC_WALL, DT_FIXED, DT_GLOBAL, DT_LOCAL,
-               FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MINMOD2,
+               FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MC2, LIM_MINMOD2,
                LIM_SMOOTHVANALBADA, LIM_VANALBADA2, LIM_VANLEER2, SCALE_ACOUSTIC, SCALE_CONVECTIVE, CaseDesc,
                Species)
 
@@ -218,6 +218,48 @@ def config_c2(nth=512, nr=256, mach=0.3, nsteps=2000):
                    "rk4", 1.0, DT_LOCAL, air_species(mach))
     return dict(name=f"euler-cylinder-ogrid-{nth}x{nr}-roe-minmod-lsq-rk4-local", nodes=nodes, q=q, case=cs,
                 bref=bref, nsteps=nsteps)
+
+
+def shocktube_states(problem="sod", species=None):
+    """Left / right primitive (rho, u, p) of the shock tubes of cases/euler/oneD/shocktubes.h:
+    Sod's (rho, u, p) = (1, 0, 1) | (0.125, 0, 0.1); the low-Mach tube (Sachdev, Hosangadi and
+    Sankaran 2012) p = 100028.04 | 100000 Pa at T = 300 K, rho = p / (R T)."""
+    if problem == "sod":
+        return (1.0, 0.0, 1.0), (0.125, 0.0, 0.1)
+    R = (species or air_species()).R
+    return (100028.04 / (R * 300.0), 0.0, 100028.04), (100000.0 / (R * 300.0), 0.0, 100000.0)
+
+
+def config_shocktube(n, flux=FLUX_ROE, limiter=LIM_MC2, t_end=0.2, nsteps=None, problem="sod", rk="ssp34",
+                     basis=BASIS_PRIMITIVE, minf=1.0):
+    """The reference's 1-D shock tube (script/euler/shocktube.cpp: MUSCL + MC2, SSP(3,4), Riemann
+    ghost cells, air species) on an n x 4 strip of square cells over [0, 1] x [0, 4/n], periodic
+    across the strip, diaphragm at x = 1/2.  Fixed step so that the physical time is t_end: the
+    update divides by the reference's volume (area / 4), so one step of dt advances 4 dt.
+    minf (the species' free-stream Mach number, the low-Mach cut-off of AUSM+up and SLAU) is 1:
+    there is no free stream, and the species default 1e-4 scales AUSM+up's pressure-diffusion
+    mass flux by ~1/minf, which at the diaphragm blows the tube up within two steps."""
+    sp = air_species(minf)
+    left, right = shocktube_states(problem, sp)
+    w = 4
+    nodes = mesh_linspace(n, w, 0.0, 1.0, 0.0, w / n)
+    x = cell_centres(nodes)[..., 0]
+    q = np.empty((n, w, 4))
+    for k, (r, u, p) in enumerate((left, right)):
+        m = (x < 0.5) if k == 0 else (x >= 0.5)
+        q[m] = (u, 0.0, r, p)
+    if nsteps is None:
+        nsteps = n
+    bref = np.empty((bref_size(n, w), 4))
+    bref[:] = (left[1], 0.0, left[0], left[2])          # boundary id 0 (x = 0); the strip's sides are periodic
+    bref[w:2 * w] = (right[1], 0.0, right[0], right[2])   # boundary id 1 (x = 1)
+    cs = make_case(LAW_EULER, basis, flux, limiter, (BC_RIEMANN, BC_RIEMANN, BC_PERIODIC, BC_PERIODIC), rk,
+                   t_end / nsteps / 4.0, DT_FIXED, sp, period_j=(0.0, w / n))
+    if basis == BASIS_CONSERVED:
+        q = prim_to_cons(q)
+        bref = prim_to_cons(bref)
+    return dict(name=f"euler-shocktube-{problem}-{n}", nodes=nodes, q=q, case=cs, bref=bref, nsteps=nsteps,
+                left=left, right=right, t_end=t_end, x=x[:, 0])
 
 
 def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):

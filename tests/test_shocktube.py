@@ -1,0 +1,83 @@
+"""Shock tubes (script/euler/shocktube.cpp, cases/euler/oneD/shocktubes.h) on the CPU: the exact
+Riemann solver used as the reference solution, the oracle's convergence to it for every Euler
+flux, and the kernels' host execution against the oracle.
+
+The reference's shock tube is 1-D; here it runs on an n x 4 strip of square cells, periodic
+across the strip (cases.config_shocktube), which leaves the flow one-dimensional.
+"""
+import numpy as np
+import pytest
+
+import pyemu
+import pyoracle as po
+import riemann_exact as rx
+
+FLUXES = {"rusanov": 1, "roe": 2, "slau": 3, "ausmpup": 4, "roe_faceframe": 5}
+
+
+@pytest.mark.parametrize("left,right,star", [
+    # E. F. Toro, Riemann Solvers and Numerical Methods for Fluid Dynamics, table 4.3:
+    # (rho, u, p) left / right, then p*, u*, rho*_L, rho*_R
+    ((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), (0.30313, 0.92745, 0.42632, 0.26557)),
+    ((1.0, -2.0, 0.4), (1.0, 2.0, 0.4), (0.00189, 0.00000, 0.02185, 0.02185)),
+    ((1.0, 0.0, 1000.0), (1.0, 0.0, 0.01), (460.894, 19.5975, 0.57506, 5.99924)),
+    ((5.99924, 19.5975, 460.894), (5.99242, -6.19633, 46.0950), (1691.64, 8.68975, 14.2823, 31.0426)),
+    ((1.0, -19.59745, 1000.0), (1.0, -19.59745, 0.01), (460.894, 0.00001, 0.57506, 5.99924)),
+])
+def test_exact_solver_matches_the_textbook_star_states(left, right, star):
+    ps, us = rx.star_state(left, right)
+    w = rx.sample(left, right, np.array([us - 1e-9, us + 1e-9]))
+    got = (ps, us, w[0, 0], w[1, 0])
+    for g, e in zip(got, star):
+        assert abs(g - e) <= 5e-5 * max(1.0, abs(e)) + 1e-5, (got, star)
+
+
+def test_exact_solution_conserves_mass_and_energy():
+    """Integrals over [-1, 1] of the sampled Sod solution at t = 0.2: mass and energy as at t = 0,
+    momentum changed by (p_L - p_R) t (no wave reaches the ends)."""
+    left, right = (1.0, 0.0, 1.0), (0.125, 0.0, 0.1)
+    t, n = 0.2, 400000
+    x = -1.0 + (np.arange(n) + 0.5) * (2.0 / n)
+    w = rx.sample(left, right, x / t)
+    r, u, p = w[:, 0], w[:, 1], w[:, 2]
+    dx = 2.0 / n
+    mass, mom, ener = r.sum() * dx, (r * u).sum() * dx, (p / 0.4 + 0.5 * r * u * u).sum() * dx
+    assert abs(mass - (1.0 + 0.125)) <= 1e-5
+    assert abs(mom - (1.0 - 0.1) * t) <= 1e-5
+    assert abs(ener - (1.0 + 0.1) / 0.4) <= 1e-5
+
+
+def _l1(cfg, q):
+    ex = rx.sample(cfg["left"], cfg["right"], (cfg["x"] - 0.5) / cfg["t_end"])
+    # primitive (u, v, rho, p) per cell against (rho, u, p)
+    return [float(np.abs(q[:, :, k] - ex[:, None, m]).mean()) for k, m in ((2, 0), (0, 1), (3, 2))]
+
+
+@pytest.mark.parametrize("flux", FLUXES.values(), ids=FLUXES.keys())
+def test_oracle_converges_to_the_exact_solution(flux):
+    """L1 errors of density, velocity and pressure at t = 0.2 for n = 100, 200, 400: MUSCL + MC2
+    is first order across the shock and the contact, so each doubling must cut them by 1.6 at
+    least, and the n = 400 density error is that of a sound scheme (below 2e-3)."""
+    from hyperboliconvergence_amd import cases
+    errs = []
+    for n in (100, 200, 400):
+        cfg = cases.config_shocktube(n, flux)
+        q, t, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+        assert abs(4.0 * t - cfg["t_end"]) <= 1e-12
+        assert np.all(np.isfinite(q))
+        errs.append(_l1(cfg, q))
+    errs = np.array(errs)
+    print(flux, errs)
+    assert np.all(errs[1:] <= errs[:-1] / 1.6)
+    assert errs[-1, 0] <= 2e-3
+
+
+def test_host_execution_of_the_kernels_is_bit_identical_to_the_oracle():
+    from hyperboliconvergence_amd import cases
+    for flux in FLUXES.values():
+        for basis in (0, 1):
+            cfg = cases.config_shocktube(64, flux, basis=basis, nsteps=40)
+            qe, _, st = pyemu.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], 40)
+            qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 40, cfg["bref"])
+            assert np.array_equal(qe, qo), (flux, basis)
+            assert st == 0
