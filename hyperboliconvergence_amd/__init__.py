@@ -15,7 +15,7 @@ LIBPATH = os.environ.get("HCAMD_LIBRARY", os.path.join(PKG, "libhcamd.so"))
 
 LAW_SCALAR, LAW_EULER = 0, 1
 BASIS_CONSERVED, BASIS_PRIMITIVE = 0, 1
-FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE, FLUX_SLAU, FLUX_AUSMPUP, FLUX_ROE_FACEFRAME = range(6)
+FLUX_CENTRAL, FLUX_RUSANOV, FLUX_ROE, FLUX_SLAU, FLUX_AUSMPUP, FLUX_ROE_FACEFRAME, FLUX_ROE_UNPREC_WS = range(7)
 SCALE_CONVECTIVE, SCALE_ACOUSTIC, SCALE_ADAPTIVE = range(3)
 (LIM_NOLIMIT1, LIM_NOLIMIT2, LIM_NOLIMIT3, LIM_VANALBADA2, LIM_MINMOD2, LIM_SUPERBEE2,
  LIM_MC2, LIM_VANLEER2, LIM_CADA3, LIM_SMOOTHVANALBADA) = range(10)

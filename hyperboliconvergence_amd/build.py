@@ -25,12 +25,12 @@ ARCH = "gfx950"
 COMMON = ["-std=c++17", "-O3", f"--offload-arch={ARCH}", "-fPIC", "-I", CSRC, "-I", INCLUDE,
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wno-unused-result"]
 
-KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(6)]
+KERNELS = [(0, 0, 0), (0, 0, 1)] + [(1, b, f) for b in (0, 1) for f in range(7)]
 
 
 def _mixed_wide(law, flux):
     """The mixed path keeps FP64 face states for these (hc_point.hpp mixed_wide)."""
-    return law == 1 and flux in (0, 1, 2, 5)
+    return law == 1 and flux in (0, 1, 2, 5, 6)
 
 
 def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0), mixed_fast_div="auto"):

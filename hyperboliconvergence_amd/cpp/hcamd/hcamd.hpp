@@ -391,6 +391,11 @@ template <LawType Law> struct RoeFluxFaceFrame : FluxInterface<RoeFluxFaceFrame<
     static_assert(Law == LawType::Euler, "RoeFluxFaceFrame needs the characteristic basis of the Euler equations");
     static constexpr int id = HC_FLUX_ROE_FACEFRAME, vscale = 0, pscale = 0;
 };
+// Roe flux in the Weiss & Smith / Li & Gu form (conservationLaws/euler/euler.h:392-405,
+// euler/fluxes/roeUnprecWS.ipp); Euler only, like the reference's
+struct RoeUnprecWS : FluxInterface<RoeUnprecWS, LawType::Euler> {
+    static constexpr int id = HC_FLUX_ROE_UNPREC_WS, vscale = 0, pscale = 0;
+};
 template <LowMachScaling VFluxScaling = LowMachScaling::Convective, LowMachScaling PFluxScaling = LowMachScaling::Acoustic>
 struct Slau : FluxInterface<Slau<VFluxScaling, PFluxScaling>, LawType::Euler> {
     static constexpr int id = HC_FLUX_SLAU;

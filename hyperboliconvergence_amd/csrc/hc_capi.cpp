@@ -183,7 +183,8 @@ bool valid_ids(const hc_case_desc& d)
     if (d.law < 0 || d.law > 1) return false;
     if (d.basis < 0 || d.basis > 1) return false;
     if (d.law == HC_LAW_SCALAR && (d.basis != HC_BASIS_CONSERVED || d.flux > HC_FLUX_RUSANOV)) return false;
-    if (d.flux < 0 || d.flux > 5) return false;
+    if (d.flux < 0 || d.flux > 6) return false;
+    if (d.law == HC_LAW_SCALAR && d.flux > HC_FLUX_RUSANOV) return false;
     if (d.limiter < 0 || d.limiter > 9) return false;
     if (d.vscale < 0 || d.vscale > 2 || d.pscale < 0 || d.pscale > 2) return false;
     for (int b = 0; b < 4; ++b) {

@@ -8,8 +8,8 @@
 
 #define HC_FOR_EACH_KERNEL(X) \
     X(0, 0, 0) X(0, 0, 1)     \
-    X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 0, 5) \
-    X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 1, 3) X(1, 1, 4) X(1, 1, 5)
+    X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 0, 5) X(1, 0, 6) \
+    X(1, 1, 0) X(1, 1, 1) X(1, 1, 2) X(1, 1, 3) X(1, 1, 4) X(1, 1, 5) X(1, 1, 6)
 
 #define HC_FOR_EACH_LAWBASIS(X) X(0, 0) X(1, 0) X(1, 1)
 

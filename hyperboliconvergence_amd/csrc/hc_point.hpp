@@ -24,7 +24,10 @@ namespace hc {
 
 enum Law { LAW_SCALAR = 0, LAW_EULER = 1 };
 enum Basis { BASIS_CONSERVED = 0, BASIS_PRIMITIVE = 1 };
-enum FluxId { FLUX_CENTRAL = 0, FLUX_RUSANOV = 1, FLUX_ROE = 2, FLUX_SLAU = 3, FLUX_AUSMPUP = 4, FLUX_ROE_FACEFRAME = 5 };
+enum FluxId {
+    FLUX_CENTRAL = 0, FLUX_RUSANOV = 1, FLUX_ROE = 2, FLUX_SLAU = 3, FLUX_AUSMPUP = 4, FLUX_ROE_FACEFRAME = 5,
+    FLUX_ROE_UNPREC_WS = 6
+};
 enum Scale { SCALE_CONVECTIVE = 0, SCALE_ACOUSTIC = 1, SCALE_ADAPTIVE = 2 };
 enum LimiterId {
     LIM_NOLIMIT1 = 0, LIM_NOLIMIT2 = 1, LIM_NOLIMIT3 = 2, LIM_VANALBADA2 = 3, LIM_MINMOD2 = 4,
@@ -34,10 +37,10 @@ enum BcId { BC_PERIODIC = 0, BC_RIEMANN = 1, BC_ENTROPY = 2, BC_FIXED = 3, BC_WA
 
 // Mixed precision keeps the solution tile, face states and stored face fluxes in FP64 where the
 // flux splits into an FP64 central part and a dissipation proportional to the jump (Euler with
-// the central, Rusanov or Roe fluxes); SLAU and AUSM+up run their faces entirely in FP32.
+// the central, Rusanov or Roe-family fluxes); SLAU and AUSM+up run their faces entirely in FP32.
 constexpr bool mixed_wide(int law, int flux)
 {
-    return law == 1 && (flux == 0 || flux == 1 || flux == 2 || flux == 5);
+    return law == 1 && (flux == 0 || flux == 1 || flux == 2 || flux == 5 || flux == 6);
 }
 
 template <class R> struct NumLimits;
@@ -529,6 +532,54 @@ HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R
     return o;
 }
 
+// euler/fluxes/roeUnprecWS.ipp:24-104 (RoeUnprecWS, the "Weiss & Smith / Li & Gu form" of
+// euler.h:392-405), dissipation part: fd = 0.5 A (|u_n| dq + dU (rho u, rho v, rho, rho H) +
+// dP (n_x, n_y, 0, u_n)) of the Roe state, dU = nu_p dp + nu_u du, dP = mu_p dp + mu_u du, with
+// the source's fixed coefficients and exponents A11 = A21 = A22 = 1, A12 = 0 on the species'
+// free-stream Mach number (pow(minf, 1) = minf, pow(minf, 0) = 1, both exact); returns the Roe
+// state's spectral radius times the area.  dp, du and the conserved jump dq come in from the caller.
+template <class R>
+HC_HD R roe_unprec_ws_dissipation(const Species<R>& sp, const FaceG<R>& f, const EState<R>& ravg, R dp, R du,
+                                  const R* dq, R* fd)
+{
+    const R una = proj_vel(f.n0, f.n1, ravg.u, ravg.v);
+    const R ra = ravg.r;
+    const R aa = hc_sqrt(ravg.a2);
+    const R minf = sp.minf;
+    const R uinf = minf * aa;
+    const R nu_p = R(1.00) * (R(1.) / (ra * uinf * uinf)) * minf;
+    const R nu_u = R(0.05);
+    const R mu_p = R(0.02) * minf;
+    const R mu_u = R(0.10) * (ra * uinf) * minf;
+    const R dU = nu_p * dp + nu_u * du;
+    const R dP = mu_p * dp + mu_u * du;
+    const R ruvh[4] = {ra * ravg.u, ra * ravg.v, ra, ra * ravg.h};
+    const R onu[4] = {f.n0, f.n1, R(0), una};
+    const R aun = hc_fabs(una);
+    for (int v = 0; v < 4; ++v) fd[v] = (R(0.5) * ((aun * dq[v] + dU * ruvh[v]) + dP * onu[v])) * f.area;
+    return euler_specrad(f.n0, f.n1, ravg) * f.area;
+}
+
+// euler/fluxes/roeUnprecWS.ipp:1-109: central flux minus the dissipation above,
+// lambda = max(central, Roe-state spectral radius * area)
+template <class R>
+HC_HD Flux4<R> euler_roe_unprec_ws(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
+{
+    const EState<R> ravg = roe_average(sp, sl, sr);
+    const Flux4<R> c = euler_central(f, sl, sr);
+    const R dp = sr.p - sl.p;
+    const R du = proj_vel(f.n0, f.n1, sr.u, sr.v) - proj_vel(f.n0, f.n1, sl.u, sl.v);
+    R cr[4], cl[4], dq[4], fd[4];
+    euler_to_conserved(sr, cr);
+    euler_to_conserved(sl, cl);
+    for (int v = 0; v < 4; ++v) dq[v] = cr[v] - cl[v];
+    const R dl = roe_unprec_ws_dissipation(sp, f, ravg, dp, du, dq, fd);
+    Flux4<R> o;
+    for (int v = 0; v < 4; ++v) o.f[v] = c.f[v] - fd[v];
+    o.lambda = hc_fmax(c.lambda, dl);
+    return o;
+}
+
 template <class R> HC_HD EState<float> state_f32(const EState<R>& s)
 {
     return EState<float>{float(s.p), float(s.r), float(s.t), float(s.h), float(s.v2), float(s.a2), float(s.u), float(s.v)};
@@ -552,6 +603,28 @@ HC_HD Flux4<double> euler_roe_mixed(const Species<float>& spf, const FaceG<doubl
     const double m = -0.5 * f.area;
     Flux4<double> o;
     for (int v = 0; v < 4; ++v) o.f[v] = c.f[v] + double(fd[v]) * m;
+    o.lambda = hc_fmax(c.lambda, double(dl));
+    return o;
+}
+
+// Mixed-precision RoeUnprecWS: central flux and the jumps (dq, dp, du) in FP64, the dissipation
+// in FP32 on the Roe state of the rounded face states
+HC_HD Flux4<double> euler_roe_unprec_ws_mixed(const Species<float>& spf, const FaceG<double>& f,
+                                              const EState<double>& sl, const EState<double>& sr)
+{
+    const Flux4<double> c = euler_central(f, sl, sr);
+    double cr[4], cl[4];
+    euler_to_conserved(sr, cr);
+    euler_to_conserved(sl, cl);
+    float dq[4], fd[4];
+    for (int v = 0; v < 4; ++v) dq[v] = float(cr[v] - cl[v]);
+    const float dp = float(sr.p - sl.p);
+    const float du = float(proj_vel(f.n0, f.n1, sr.u, sr.v) - proj_vel(f.n0, f.n1, sl.u, sl.v));
+    const FaceG<float> ff{float(f.area), float(f.n0), float(f.n1)};
+    const EState<float> ravg = roe_average(spf, state_f32(sl), state_f32(sr));
+    const float dl = roe_unprec_ws_dissipation(spf, ff, ravg, dp, du, dq, fd);
+    Flux4<double> o;
+    for (int v = 0; v < 4; ++v) o.f[v] = c.f[v] - double(fd[v]);
     o.lambda = hc_fmax(c.lambda, double(dl));
     return o;
 }
@@ -693,12 +766,13 @@ HC_HD Flux4<R> face_flux(const Species<R>& sp, int vscale, int pscale, const Fac
         else if constexpr (FLUX == FLUX_ROE) return euler_roe(sp, f, sl, sr);
         else if constexpr (FLUX == FLUX_SLAU) return euler_slau(sp, f, sl, sr);
         else if constexpr (FLUX == FLUX_ROE_FACEFRAME) return euler_roe<R, true>(sp, f, sl, sr);
+        else if constexpr (FLUX == FLUX_ROE_UNPREC_WS) return euler_roe_unprec_ws(sp, f, sl, sr);
         else return euler_ausmpup(sp, vscale, pscale, f, sl, sr);
     }
 }
 
-// Mixed-precision face flux on FP64 face states: the Roe family splits into an FP64 central flux
-// and an FP32 dissipation; Rusanov and central are cheap and stay FP64; SLAU and AUSM+up, which
+// Mixed-precision face flux on FP64 face states: the Roe family (Roe, face-frame Roe,
+// RoeUnprecWS) splits into an FP64 central flux and an FP32 dissipation; Rusanov and central are cheap and stay FP64; SLAU and AUSM+up, which
 // do not split that way, run in FP32 on the rounded face states.
 template <int LAW, int BASIS, int FLUX>
 HC_HD Flux4<double> face_flux_mixed(const Species<double>& sp, const Species<float>& spf, int vscale, int pscale,
@@ -708,6 +782,10 @@ HC_HD Flux4<double> face_flux_mixed(const Species<double>& sp, const Species<flo
         const EState<double> sl = euler_state<double, BASIS>(sp, ql);
         const EState<double> sr = euler_state<double, BASIS>(sp, qr);
         return euler_roe_mixed<FLUX == FLUX_ROE_FACEFRAME>(spf, f, sl, sr);
+    } else if constexpr (LAW == LAW_EULER && FLUX == FLUX_ROE_UNPREC_WS) {
+        const EState<double> sl = euler_state<double, BASIS>(sp, ql);
+        const EState<double> sr = euler_state<double, BASIS>(sp, qr);
+        return euler_roe_unprec_ws_mixed(spf, f, sl, sr);
     } else if constexpr (FLUX == FLUX_RUSANOV || FLUX == FLUX_CENTRAL) {
         return face_flux<double, LAW, BASIS, FLUX>(sp, vscale, pscale, f, ql, qr);
     } else {

@@ -37,9 +37,10 @@ extern "C" {
 /* identifiers (same numbering as the oracle and the C++ drop-in layer) */
 enum { HC_LAW_SCALAR = 0, HC_LAW_EULER = 1 };
 enum { HC_BASIS_CONSERVED = 0, HC_BASIS_PRIMITIVE = 1 };
-/* HC_FLUX_ROE_FACEFRAME: Roe with the Roe-average velocity in the face frame (no reference counterpart) */
+/* HC_FLUX_ROE_FACEFRAME: Roe with the Roe-average velocity in the face frame (no reference counterpart);
+   HC_FLUX_ROE_UNPREC_WS: RoeUnprecWS (conservationLaws/euler/euler.h:392-405, euler/fluxes/roeUnprecWS.ipp) */
 enum { HC_FLUX_CENTRAL = 0, HC_FLUX_RUSANOV = 1, HC_FLUX_ROE = 2, HC_FLUX_SLAU = 3, HC_FLUX_AUSMPUP = 4,
-       HC_FLUX_ROE_FACEFRAME = 5 };
+       HC_FLUX_ROE_FACEFRAME = 5, HC_FLUX_ROE_UNPREC_WS = 6 };
 enum { HC_SCALE_CONVECTIVE = 0, HC_SCALE_ACOUSTIC = 1, HC_SCALE_ADAPTIVE = 2 };
 enum { HC_LIM_NOLIMIT1 = 0, HC_LIM_NOLIMIT2 = 1, HC_LIM_NOLIMIT3 = 2, HC_LIM_VANALBADA2 = 3,
        HC_LIM_MINMOD2 = 4, HC_LIM_SUPERBEE2 = 5, HC_LIM_MC2 = 6, HC_LIM_VANLEER2 = 7,

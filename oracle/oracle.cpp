@@ -582,6 +582,46 @@ FluxRes roe_flux(const orc_case& cs, const Species& sp, const Face& f, const dou
     return out;
 }
 
+// euler/fluxes/roeUnprecWS.ipp:1-109 (RoeUnprecWS, "Weiss & Smith / Li & Gu form", euler.h:392-405).
+// Roe state of the two face states; flux = central - 0.5 (|u_n| dq + dU ruvh + dP onu) A, where
+// dq is the conserved jump, ruvh = rho (u, v, 1, H) and onu = (n, 0, u_n) of the Roe state,
+// dU = nu_p dp + nu_u du and dP = mu_p dp + mu_u du with the jumps of pressure and normal velocity
+// and the coefficients of roeUnprecWS.ipp:61-69 (exponents A11 = 1, A12 = 0, A21 = 1, A22 = 1 on
+// the species' free-stream Mach number, uinf = minf a); lambda = max(central, Roe-state spectral
+// radius * area * minf^(A11 - 1)).
+FluxRes roe_unprec_ws_flux(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr)
+{
+    double ravg[8];
+    roe_average(sp, sl, sr, ravg);
+    const FluxRes central = central_flux(cs, f, sl, sr);
+    const double dp = sr[SP] - sl[SP];
+    const double du = projected_velocity(f.n, sr, SU) - projected_velocity(f.n, sl, SU);
+    const double una = projected_velocity(f.n, ravg, SU);
+    const double ra = ravg[SR];
+    const double aa = std::sqrt(ravg[SA2]);
+    const double minf = sp.minf;
+    const double uinf = minf * aa;
+    double cr[4], cl[4];
+    euler_state2cons(sr, cr);
+    euler_state2cons(sl, cl);
+    const int A11 = 1, A12 = 0, A21 = 1, A22 = 1;
+    const double nu_p = 1.00 * (1. / (ra * uinf * uinf)) * std::pow(minf, A11);
+    const double nu_u = 0.05 * std::pow(minf, A12);
+    const double mu_p = 0.02 * std::pow(minf, A21);
+    const double mu_u = 0.10 * (ra * uinf) * std::pow(minf, A22);
+    const double deltaU = nu_p * dp + nu_u * du;
+    const double deltaP = mu_p * dp + mu_u * du;
+    const double ruvh[4] = {ra * ravg[SU], ra * ravg[SU + 1], ra, ra * ravg[SH]};
+    const double onu[4] = {f.n[0], f.n[1], 0., una};
+    FluxRes out{};
+    for (int v = 0; v < 4; ++v) {
+        const double diff = 0.5 * (std::fabs(una) * (cr[v] - cl[v]) + deltaU * ruvh[v] + deltaP * onu[v]) * f.area;
+        out.f[v] = central.f[v] - diff;
+    }
+    out.lambda = std::fmax(central.lambda, spectral_radius(cs, f.n, ravg) * f.area * std::pow(minf, A11 - 1));
+    return out;
+}
+
 // euler/fluxes/machSplittings.ipp:179-214
 inline double msM1(int sign, double M) { return 0.5 * (M + sign * std::fabs(M)); }
 inline double msM2(int sign, double M) { return sign * 0.25 * (M + sign) * (M + sign); }
@@ -708,6 +748,7 @@ FluxRes flux_states(const orc_case& cs, const Species& sp, const Face& f, const 
     case ORC_FLUX_ROE: return roe_flux(cs, sp, f, sl, sr);
     case ORC_FLUX_SLAU: return slau_flux(cs, sp, f, sl, sr);
     case ORC_FLUX_ROE_FACEFRAME: return roe_flux(cs, sp, f, sl, sr, true);
+    case ORC_FLUX_ROE_UNPREC_WS: return roe_unprec_ws_flux(cs, sp, f, sl, sr);
     default: return ausm_flux(cs, sp, f, sl, sr);
     }
 }
@@ -1274,7 +1315,7 @@ bool valid_case(const orc_case& cs, int ni, int nj)
     if ((cs.bc[0] == ORC_BC_PERIODIC) != (cs.bc[1] == ORC_BC_PERIODIC)) return false;
     if ((cs.bc[2] == ORC_BC_PERIODIC) != (cs.bc[3] == ORC_BC_PERIODIC)) return false;
     if (cs.law == ORC_LAW_SCALAR && cs.flux > ORC_FLUX_RUSANOV) return false;
-    if (cs.flux < 0 || cs.flux > ORC_FLUX_ROE_FACEFRAME) return false;
+    if (cs.flux < 0 || cs.flux > ORC_FLUX_ROE_UNPREC_WS) return false;
     if (cs.nstages < 1 || cs.nstages > 6) return false;
     if (cs.dt_mode < ORC_DT_GLOBAL || cs.dt_mode > ORC_DT_FIXED) return false;
     return true;

@@ -33,7 +33,8 @@ typedef struct {
 enum { ORC_LAW_SCALAR = 0, ORC_LAW_EULER = 1 };
 enum { ORC_BASIS_CONSERVED = 0, ORC_BASIS_PRIMITIVE = 1 };
 enum { ORC_FLUX_CENTRAL = 0, ORC_FLUX_RUSANOV = 1, ORC_FLUX_ROE = 2,
-       ORC_FLUX_SLAU = 3, ORC_FLUX_AUSMPUP = 4, ORC_FLUX_ROE_FACEFRAME = 5 };
+       ORC_FLUX_SLAU = 3, ORC_FLUX_AUSMPUP = 4, ORC_FLUX_ROE_FACEFRAME = 5,
+       ORC_FLUX_ROE_UNPREC_WS = 6 };
 enum { ORC_SCALE_CONVECTIVE = 0, ORC_SCALE_ACOUSTIC = 1, ORC_SCALE_ADAPTIVE = 2 };
 enum { ORC_LIM_NOLIMIT1 = 0, ORC_LIM_NOLIMIT2 = 1, ORC_LIM_NOLIMIT3 = 2,
        ORC_LIM_VANALBADA2 = 3, ORC_LIM_MINMOD2 = 4, ORC_LIM_SUPERBEE2 = 5,

@@ -35,6 +35,8 @@ int main()
         case 1: eval(RusanovFlux<Law>{}, sp, s, l, r); break;
         case 2: eval(RoeFlux<Law>{}, sp, s, l, r); break;
         case 3: eval(Slau<>{}, sp, s, l, r); break;
+        case 5: eval(RoeFluxFaceFrame<Law>{}, sp, s, l, r); break;
+        case 6: eval(RoeUnprecWS{}, sp, s, l, r); break;
         default: eval(AusmPlusUP<>{}, sp, s, l, r); break;
         }
         // the limiter functor on the first two components

@@ -136,12 +136,14 @@ template <class RF, int TI, int TJ, int NT, int CST, int GPF> void dispatch_stag
     case 103: run_stage<1, 0, 3, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 104: run_stage<1, 0, 4, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 105: run_stage<1, 0, 5, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 106: run_stage<1, 0, 6, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 110: run_stage<1, 1, 0, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 111: run_stage<1, 1, 1, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 112: run_stage<1, 1, 2, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 113: run_stage<1, 1, 3, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 114: run_stage<1, 1, 4, RF, TI, TJ, NT, CST, GPF>(a); break;
     case 115: run_stage<1, 1, 5, RF, TI, TJ, NT, CST, GPF>(a); break;
+    case 116: run_stage<1, 1, 6, RF, TI, TJ, NT, CST, GPF>(a); break;
     default: break;
     }
 }

@@ -23,8 +23,8 @@ def test_dropin_point_functors_bit_identical_to_oracle():
     exe = _build_points()
     rng = np.random.default_rng(21)
     lines, cases = [], []
-    for k in range(60):
-        fid = k % 5
+    for k in range(70):
+        fid = k % 7
         lid = (4, 3, 7)[k % 3]
         n = rng.standard_normal(2)
         n /= np.linalg.norm(n)

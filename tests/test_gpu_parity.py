@@ -78,7 +78,7 @@ def test_scalar_residual_matches_oracle(limiter):
     print("bit-exact:", np.array_equal(r, ro))
 
 
-@pytest.mark.parametrize("flux", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("flux", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("basis", [0, 1])
 def test_euler_ogrid_residual_matches_oracle(flux, basis):
     cfg = _cyl_cfg(flux=flux, basis=basis, perturb=0.02)

@@ -11,7 +11,7 @@ from conftest import rel_err
 pytestmark = pytest.mark.gpu
 
 TRAJ_TOL = 1e-10
-FLUXES = {"rusanov": 1, "roe": 2, "slau": 3, "ausmpup": 4, "roe_faceframe": 5}
+FLUXES = {"rusanov": 1, "roe": 2, "slau": 3, "ausmpup": 4, "roe_faceframe": 5, "roe_unprec_ws": 6}
 
 
 def _run(cfg, precision=0):

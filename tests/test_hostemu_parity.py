@@ -37,7 +37,7 @@ def _ogrid(nr, nth, basis, flux, limiter, bc=(po.BC_WALL, po.BC_RIEMANN, 0, 0), 
     return cs, nodes, q, bref
 
 
-@pytest.mark.parametrize("flux", range(5))
+@pytest.mark.parametrize("flux", range(7))
 @pytest.mark.parametrize("basis", [0, 1])
 def test_euler_residual_and_steps_bit_identical(flux, basis):
     cs, nodes, q, bref = _ogrid(13, 37, basis, flux, po.LIM_MINMOD2)   # ragged: not multiples of the tile

@@ -106,7 +106,8 @@ def test_characteristic_waves_are_eigenvectors():
 
 # ---------------- fluxes ----------------
 
-@pytest.mark.parametrize("flux", [po.FLUX_CENTRAL, po.FLUX_RUSANOV, po.FLUX_ROE, po.FLUX_SLAU, po.FLUX_AUSMPUP])
+@pytest.mark.parametrize("flux", [po.FLUX_CENTRAL, po.FLUX_RUSANOV, po.FLUX_ROE, po.FLUX_SLAU, po.FLUX_AUSMPUP,
+                                  po.FLUX_ROE_FACEFRAME, po.FLUX_ROE_UNPREC_WS])
 def test_consistency_F_qq_is_exact_flux(flux):
     cs = euler_case(flux=flux, minf=0.5)
     rng = np.random.default_rng(3)
@@ -156,6 +157,67 @@ def test_roe_upwind_property_on_x_faces():
     assert np.allclose(F[:4], exact_flux_np(ql, (1, 0)), rtol=1e-13)
     F = po.flux(cs, face((1, 0)), prim_state(-2.5, 0.3, 1.0, 1 / G), prim_state(-2.4, 0.2, 0.9, 0.95 / G))
     assert np.allclose(F[:4], exact_flux_np(prim_state(-2.4, 0.2, 0.9, 0.95 / G), (1, 0)), rtol=1e-13)
+
+
+def test_roe_unprec_ws_moving_contact_is_upwind():
+    """RoeUnprecWS (euler/fluxes/roeUnprecWS.ipp) on a contact (equal velocity and pressure,
+    different density): the pressure and velocity perturbations vanish and |u_n| dq turns the
+    central flux into the exact flux of the upwind state, on any face and for either sign."""
+    rng = np.random.default_rng(21)
+    for minf in (0.05, 0.5):
+        cs = euler_case(flux=po.FLUX_ROE_UNPREC_WS, minf=minf)
+        for _ in range(10):
+            u, v = rng.uniform(-0.6, 0.6, 2)
+            p = rng.uniform(0.5, 1.5)
+            ql, qr = prim_state(u, v, rng.uniform(0.5, 2.0), p), prim_state(u, v, rng.uniform(0.5, 2.0), p)
+            f = face(rng.standard_normal(2), 0.9)
+            un = u * f.nx + v * f.ny
+            F = po.flux(cs, f, ql, qr)
+            up = ql if un > 0 else qr
+            assert np.allclose(F[:4], exact_flux_np(up, (f.nx, f.ny), 0.9), rtol=1e-12, atol=1e-13)
+
+
+def test_roe_unprec_ws_perturbations_scale_with_the_free_stream_mach_number():
+    """RoeUnprecWS's perturbation coefficients carry minf to the exponents A11 = 1, A12 = 0,
+    A21 = 1, A22 = 1 with uinf = minf a (roeUnprecWS.ipp:61-69).  At rest with a pressure jump
+    only: the mass and energy dissipation go as 1/minf (nu_p) and the normal-momentum one as
+    minf (mu_p); with a symmetric normal-velocity jump only (Roe velocity 0): the mass one does
+    not depend on minf (nu_u) and the normal-momentum one goes as minf^2 (mu_u); mass flows from
+    high to low pressure."""
+    def fl(minf, ql, qr):
+        return po.flux(euler_case(flux=po.FLUX_ROE_UNPREC_WS, minf=minf), face((1, 0), 0.7), ql, qr)
+
+    ql, qr = prim_state(0, 0, 1.0, 1.0), prim_state(0, 0, 1.0, 1.3)
+    Fc = po.flux(euler_case(flux=po.FLUX_CENTRAL), face((1, 0), 0.7), ql, qr)
+    ms = (0.05, 0.1, 0.4)
+    F = [fl(m, ql, qr) for m in ms]
+    assert all(f[2] < 0 for f in F)
+    for m, f in zip(ms, F):
+        assert math.isclose(f[2] * m, F[0][2] * ms[0], rel_tol=1e-13)
+        assert math.isclose(f[3] * m, F[0][3] * ms[0], rel_tol=1e-13)
+        # differences of O(1e-4) from O(1) fluxes: 1e-9 is their round-off
+        assert math.isclose((f[0] - Fc[0]) / m, (F[0][0] - Fc[0]) / ms[0], rel_tol=1e-9)
+        assert f[1] == 0.0
+    # the coefficients themselves (roeUnprecWS.ipp:66-69: 1.00, 0.05, 0.02, 0.10): equal densities
+    # at rest give the Roe state H = (H_L + H_R) / 2, a^2 = (gamma - 1) H, u = 0
+    H = 0.5 * (G / (G - 1) * 1.0 + G / (G - 1) * 1.3)
+    a2 = (G - 1) * H
+    dp, A = 0.3, 0.7
+    for m, f in zip(ms, F):
+        assert math.isclose(f[2], -0.5 * 1.00 * dp / (m * a2) * A, rel_tol=1e-13)    # nu_p rho
+        assert math.isclose(f[3] / f[2], H, rel_tol=1e-13)                         # ruvh = rho (u, v, 1, H)
+        assert math.isclose(f[0] - Fc[0], -0.5 * 0.02 * m * dp * A, rel_tol=1e-9)   # mu_p n_x
+    ql, qr = prim_state(0.1, 0, 1.0, 1.0), prim_state(-0.1, 0, 1.0, 1.0)
+    Fc = po.flux(euler_case(flux=po.FLUX_CENTRAL), face((1, 0), 0.7), ql, qr)
+    F = [fl(m, ql, qr) for m in ms]
+    H = G / (G - 1) * 1.0 + 0.5 * 0.1 ** 2
+    a = math.sqrt((G - 1) * H)   # Roe velocity 0: a^2 = (gamma - 1) H
+    du = -0.2
+    for m, f in zip(ms, F):
+        assert math.isclose(f[2], F[0][2], rel_tol=1e-14)
+        assert math.isclose((f[0] - Fc[0]) / m ** 2, (F[0][0] - Fc[0]) / ms[0] ** 2, rel_tol=1e-9)
+        assert math.isclose(f[2], -0.5 * 0.05 * du * A, rel_tol=1e-13)                  # nu_u rho
+        assert math.isclose(f[0] - Fc[0], -0.5 * 0.10 * (m * a) * m * du * A, rel_tol=1e-9)   # mu_u n_x
 
 
 def test_roe_rest_states_rotate_and_frame_mixing_quirk():

@@ -12,7 +12,7 @@ import pyemu
 import pyoracle as po
 import riemann_exact as rx
 
-FLUXES = {"rusanov": 1, "roe": 2, "slau": 3, "ausmpup": 4, "roe_faceframe": 5}
+FLUXES = {"rusanov": 1, "roe": 2, "slau": 3, "ausmpup": 4, "roe_faceframe": 5, "roe_unprec_ws": 6}
 
 
 @pytest.mark.parametrize("left,right,star", [
