@@ -11,7 +11,7 @@ import math
 import numpy as np
 
 from . import (BASIS_CONSERVED, BASIS_PRIMITIVE, BC_PERIODIC, BC_RIEMANN, BC_WALL, DT_FIXED, DT_GLOBAL, DT_LOCAL,
-               FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MC2, LIM_MINMOD2,
+               FLUX_AUSMPUP, FLUX_ROE, FLUX_RUSANOV, FLUX_SLAU, LAW_EULER, LAW_SCALAR, LIM_MC2, LIM_MINMOD2, LIM_NOLIMIT3,
                LIM_SMOOTHVANALBADA, LIM_VANALBADA2, LIM_VANLEER2, SCALE_ACOUSTIC, SCALE_CONVECTIVE, CaseDesc,
                Species)
 
@@ -260,6 +260,46 @@ def config_shocktube(n, flux=FLUX_ROE, limiter=LIM_MC2, t_end=0.2, nsteps=None, 
         bref = prim_to_cons(bref)
     return dict(name=f"euler-shocktube-{problem}-{n}", nodes=nodes, q=q, case=cs, bref=bref, nsteps=nsteps,
                 left=left, right=right, t_end=t_end, x=x[:, 0])
+
+
+def gresho_state(x, y, mach=0.3, vel_inf=0.1, density=1.0, gamma=GAMMA):
+    """Gresho vortex (script/euler/gresho2D.cpp:163-205) advected at vel_inf along x: azimuthal
+    velocity 5r for r < 0.2, 2 - 5r for r < 0.4, 0 outside; pressure p0 + 12.5 r^2, then
+    p0 + 12.5 r^2 + 4 (1 - 5r - ln 0.2 + ln r), then p0 - 2 + 4 ln 2, p0 = rho / (gamma M^2),
+    so that dp/dr = rho u_theta^2 / r.  Primitive {u, v, rho, p}; r must not vanish."""
+    x, y = np.asarray(x, float), np.asarray(y, float)
+    p0 = density / (gamma * mach * mach)
+    r = np.sqrt(x * x + y * y)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        lnr = np.log(r)
+    inner, ring = r < 0.2, (r >= 0.2) & (r < 0.4)
+    ut = np.where(inner, 5.0 * r, np.where(ring, 2.0 - 5.0 * r, 0.0))
+    p = np.where(inner, p0 + 12.5 * r * r,
+                 np.where(ring, p0 + 12.5 * r * r + 4.0 * (1.0 - 5.0 * r - math.log(0.2) + lnr),
+                          p0 - 2.0 + 4.0 * math.log(2.0)))
+    sin_t, cos_t = y / r, x / r
+    q = np.empty(x.shape + (4,))
+    q[..., 0] = vel_inf - ut * sin_t
+    q[..., 1] = ut * cos_t
+    q[..., 2] = density
+    q[..., 3] = p
+    return q
+
+
+def config_gresho(n=256, flux=FLUX_RUSANOV, limiter=LIM_NOLIMIT3, nsteps=10, cfl=1.6, mach=0.3, vel_inf=0.1,
+                  density=1.0):
+    """The reference's Gresho driver (script/euler/gresho2D.cpp:68-110, 156-224): n x n cells over
+    [-1, 1]^2, periodic, Rusanov + NoLimit3 MUSCL, SSP(3,4), global dt at CFL 1.6, species minf = M.
+    qref is the outer (rest) state used for the boundary fill and the output's reference values."""
+    nodes = mesh_linspace(n, n, -1.0, 1.0, -1.0, 1.0)
+    c = cell_centres(nodes)
+    q = gresho_state(c[..., 0], c[..., 1], mach, vel_inf, density)
+    p0 = density / (GAMMA * mach * mach)
+    qref = np.array([0.0, 0.0, density, p0 - 2.0 + 4.0 * math.log(2.0)])
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, flux, limiter, (BC_PERIODIC,) * 4, "ssp34", cfl, DT_GLOBAL,
+                   air_species(mach), period_i=(2.0, 0.0), period_j=(0.0, 2.0))
+    return dict(name=f"euler-gresho-{n}x{n}", nodes=nodes, q=q, case=cs, bref=np.broadcast_to(qref, (bref_size(n, n), 4)).copy(),
+                qref=qref, nsteps=nsteps)
 
 
 def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):

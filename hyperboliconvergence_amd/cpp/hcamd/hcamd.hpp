@@ -355,6 +355,53 @@ template <auto Basis> constexpr int basis_id()
 template <typename E> constexpr int scale_id(E s) { return s == E::Convective ? 0 : s == E::Acoustic ? 1 : 2; }
 }   // namespace hcamd_detail
 
+// State<Law,nDim,Real>: the derived point-wise quantities of a solution set with the reference's
+// accessors (conservationLaws/euler/euler.h:62-83: p, rho, T, H, |u|^2, a^2, u_i;
+// scalarAdvection/scalarAdvection.h:48-59: q, u_i), so drivers' output code (writeState of
+// script/euler/cylinder2D.cpp:35-63, gresho2D.cpp, shocktube.cpp, script/scalar/cartesian2D.cpp)
+// compiles unchanged.  set2State evaluates them with the kernels' point functions.
+template <LawType Law, int nDim, floating_point Real> struct State;
+template <int nDim, floating_point Real> struct State<LawType::Euler, nDim, Real> {
+    std::array<Real, nDim + 6> state{};
+    Real& pressure() { return state[0]; }
+    Real& density() { return state[1]; }
+    Real& temperature() { return state[2]; }
+    Real& specificTotalEnthalpy() { return state[3]; }
+    Real& velocity2() { return state[4]; }
+    Real& speedOfSound2() { return state[5]; }
+    Real& velocity(const int i) { return state[6 + i]; }
+    const Real& pressure() const { return state[0]; }
+    const Real& density() const { return state[1]; }
+    const Real& temperature() const { return state[2]; }
+    const Real& specificTotalEnthalpy() const { return state[3]; }
+    const Real& velocity2() const { return state[4]; }
+    const Real& speedOfSound2() const { return state[5]; }
+    const Real& velocity(const int i) const { return state[6 + i]; }
+};
+template <int nDim, floating_point Real> struct State<LawType::ScalarAdvection, nDim, Real> {
+    std::array<Real, nVar<LawType::ScalarAdvection, nDim>> state{};
+    Real& scalar() { return state[0]; }
+    const Real& scalar() const { return state[0]; }
+    Real& velocity(const int i) { return state[1 + i]; }
+    const Real& velocity(const int i) const { return state[1 + i]; }
+};
+
+// set2State (euler/transforms/{conserved,primitive}.ipp, scalarAdvection/transforms/conserved.ipp)
+template <LawType Law, int nDim, auto Basis, floating_point Real>
+State<Law, nDim, Real> set2State(const Species<Law, Real>& species, const VariableSet<Law, nDim, Basis, Real>& q)
+{
+    State<Law, nDim, Real> st;
+    if constexpr (Law == LawType::Euler) {
+        const hc::EState<Real> s = hc::euler_state<Real, hcamd_detail::basis_id<Basis>()>(hcamd_detail::species(species),
+                                                                                        q.elems.data());
+        st.state = {s.p, s.r, s.t, s.h, s.v2, s.a2, s.u, s.v};
+    } else {
+        const hc::SState<Real> s = hc::scalar_state(q.elems.data());
+        st.state = {s.q, s.u, s.v};
+    }
+    return st;
+}
+
 // FluxInterface CRTP: any solution basis -> FluxResult in the conserved basis (base/base.h:370-397)
 template <typename Flux, LawType Law> struct FluxInterface {
     template <int nDim, auto Basis, floating_point Real>

@@ -49,6 +49,10 @@ int main()
         default: z = Limiters::VanLeer2{}(a, b); break;
         }
         std::printf("%.17e %.17e %.17e %.17e\n", z[0], z[1], z[2], z[3]);
+        // set2State of the left set with the reference's accessors
+        const auto st = set2State(sp, l);
+        std::printf("%.17e %.17e %.17e %.17e %.17e %.17e %.17e %.17e\n", st.pressure(), st.density(), st.temperature(),
+                    st.specificTotalEnthalpy(), st.velocity2(), st.speedOfSound2(), st.velocity(0), st.velocity(1));
     }
     return 0;
 }

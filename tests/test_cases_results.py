@@ -1,0 +1,111 @@
+"""The case layer beyond the benchmark configurations: the Gresho vortex of script/euler/gresho2D.cpp
+(its profile pinned by radial equilibrium and continuity, its low-Mach behaviour on the oracle)
+and the result writers with the reference drivers' column formats."""
+import math
+import re
+
+import numpy as np
+
+import pyoracle as po
+
+NUM8 = re.compile(r"^-?\d\.\d{8}e[+-]\d{2,3}$")
+NUM16 = re.compile(r"^-?\d\.\d{16}e[+-]\d{2,3}$")
+
+
+def test_gresho_profile_is_in_radial_equilibrium_and_continuous():
+    """dp/dr = rho u_theta^2 / r (central differences of the profile), continuity of u_theta and p
+    at r = 0.2 and 0.4, rest (plus the advection velocity) outside r = 0.4."""
+    from hyperboliconvergence_amd import cases
+    vel = 0.1
+    for r in (0.05, 0.15, 0.25, 0.35, 0.45, 0.8):
+        for th in (0.3, 2.0, 4.5):
+            h = 1e-5
+            x, y = r * math.cos(th), r * math.sin(th)
+            q = cases.gresho_state(np.array([x]), np.array([y]), vel_inf=vel)[0]
+            ut = -(q[0] - vel) * math.sin(th) + q[1] * math.cos(th)
+            ur = (q[0] - vel) * math.cos(th) + q[1] * math.sin(th)
+            assert abs(ur) <= 1e-15
+            qp = cases.gresho_state(np.array([(r + h) * math.cos(th)]), np.array([(r + h) * math.sin(th)]), vel_inf=vel)[0]
+            qm = cases.gresho_state(np.array([(r - h) * math.cos(th)]), np.array([(r - h) * math.sin(th)]), vel_inf=vel)[0]
+            dpdr = (qp[3] - qm[3]) / (2 * h)
+            assert abs(dpdr - q[2] * ut * ut / r) <= 1e-6, (r, dpdr, ut)
+            if r > 0.4:
+                assert ut == 0.0 and q[0] == vel
+    for rs in (0.2, 0.4):
+        a = cases.gresho_state(np.array([rs - 1e-13]), np.array([0.0]))[0]
+        b = cases.gresho_state(np.array([rs + 1e-13]), np.array([0.0]))[0]
+        assert np.allclose(a, b, rtol=0, atol=1e-11)
+
+
+def test_gresho_low_mach_dissipation_of_the_fluxes():
+    """At M = 0.03 (64^2, 100 steps of the reference driver's SSP(3,4) at CFL 1.6 with its NoLimit3
+    MUSCL) RoeUnprecWS, whose dissipation scales with the free-stream Mach number, keeps 99.9 % of
+    the vortex's kinetic energy, the face-frame Roe 98.6 % and Rusanov, whose dissipation scales
+    with the speed of sound, 94 % (with MinMod2: 99.5, 89 and 63 %)."""
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+
+    def ke_ratio(flux):
+        cfg = cases.config_gresho(64, flux=flux, mach=0.03, nsteps=100)
+        q, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+        ke = lambda s: float((s[..., 2] * ((s[..., 0] - 0.1) ** 2 + s[..., 1] ** 2)).sum())
+        return ke(q) / ke(cfg["q"])
+
+    unprec, ff, rus = ke_ratio(hca.FLUX_ROE_UNPREC_WS), ke_ratio(hca.FLUX_ROE_FACEFRAME), ke_ratio(hca.FLUX_RUSANOV)
+    print("kinetic energy kept:", unprec, ff, rus)
+    assert unprec >= 0.998 and ff >= 0.98 and rus <= 0.95
+    assert unprec > ff > rus
+
+
+def _lines(path):
+    return open(path).read().split("\n")
+
+
+def test_cylinder_and_mesh_writers(tmp_path):
+    from hyperboliconvergence_amd import cases, results
+    cfg = cases.config_c2(nth=16, nr=8)
+    qref = cfg["q"][0, 0]
+    results.write_cylinder_result(tmp_path / "r.dat", cfg["nodes"], cfg["q"], qref)
+    lines = _lines(tmp_path / "r.dat")
+    assert lines.count("") == 8 + 1          # a blank line after each ring, then the end of file
+    rows = [l.split(" ") for l in lines if l]
+    assert len(rows) == 8 * 16 and all(len(r) == 9 and all(NUM8.match(x) for x in r) for r in rows)
+    a = np.loadtxt(tmp_path / "r.dat")
+    assert np.allclose(a[:16, 0], (np.arange(16) + 0.5) * 2 * math.pi / 16, rtol=1e-8)
+    assert np.all(a[:, 1] == 0.0)
+    # the free stream itself: relative columns 0, u/|u0| = 1, v = 0, a/a0 = 1
+    assert np.allclose(a[:, 3:], [0, 1, 0, 0, 0, 1], atol=1e-15)
+    assert np.all(np.diff(a[::16, 2]) > 0)   # radius grows outwards ring by ring
+    results.write_mesh(tmp_path / "m.dat", cfg["nodes"])
+    m = np.loadtxt(tmp_path / "m.dat")
+    assert np.allclose(m, cfg["nodes"].reshape(-1, 2), rtol=1e-8, atol=1e-12)
+    assert _lines(tmp_path / "m.dat").count("") == 9 + 1
+
+
+def test_gresho_shocktube_and_scalar_writers(tmp_path):
+    from hyperboliconvergence_amd import cases, results
+    g = cases.config_gresho(8)
+    q = np.broadcast_to(g["qref"], g["q"].shape).copy()
+    results.write_gresho_result(tmp_path / "g.dat", g["nodes"], q, g["qref"])
+    rows = [l.split(" ") for l in _lines(tmp_path / "g.dat") if l]
+    assert len(rows) == 64 and all(len(r) == 9 and all(NUM16.match(x) for x in r) for r in rows)
+    a = np.loadtxt(tmp_path / "g.dat")
+    assert np.allclose(a[:, 2:], [1, 0, 0, 0, 1, 1, 0], atol=1e-15)
+    assert np.allclose(a[:, :2], cases.cell_centres(g["nodes"]).reshape(-1, 2), rtol=1e-15)
+
+    st = cases.config_shocktube(10)
+    results.write_shocktube_result(tmp_path / "s.dat", st["q"])
+    s = np.loadtxt(tmp_path / "s.dat")
+    assert s.shape == (10, 5)
+    rho, p = st["q"][:, 0, 2], st["q"][:, 0, 3]
+    # at rest: u/a = u = 0; entropy a^2 / (gamma rho^(gamma-1)) = p / rho^gamma
+    assert np.allclose(s, np.stack([rho, 0 * rho, 0 * rho, p, p / rho ** 1.4], axis=1), rtol=1e-8, atol=1e-300)
+
+    c = cases.config_c1(6)
+    results.write_scalar_result(tmp_path / "c.dat", c["nodes"], c["q"])
+    lines = _lines(tmp_path / "c.dat")
+    assert lines.count("") == 6 + 1
+    first = lines[0]
+    assert re.match(r"^\S+ \S+  \S+ \S+ \S+$", first)   # two spaces after y, as the reference
+    a = np.loadtxt(tmp_path / "c.dat")
+    assert np.allclose(a[:, 2:], c["q"].reshape(-1, 3), rtol=1e-8, atol=1e-300)

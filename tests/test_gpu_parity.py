@@ -156,6 +156,21 @@ def test_global_time_step_every_rk_table_matches_oracle(rk):
         assert abs(t - to) <= 1e-12 * abs(to), graph_steps
 
 
+def test_gresho_reference_driver_matches_oracle():
+    """script/euler/gresho2D.cpp's settings: 256^2, Rusanov + NoLimit3, SSP(3,4), global dt at
+    CFL 1.6, 10 steps, M = 0.3."""
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_gresho(256)
+    with _ctx(cfg) as ctx:
+        ctx.step(cfg["nsteps"])
+        q = ctx.get_state()
+        t = ctx.time()
+        assert ctx.status() == 0
+    qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+    assert rel_err(q, qo) <= TRAJ_TOL
+    assert abs(t - to) <= 1e-12 * abs(to)
+
+
 def test_graph_replay_matches_eager():
     cfg = _cyl_cfg(64, 32)
     with _ctx(cfg) as a:
