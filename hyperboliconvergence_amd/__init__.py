@@ -26,7 +26,7 @@ STATUS_NONFINITE, STATUS_NONPHYSICAL, STATUS_BCNORMAL = 1, 2, 4
 
 C_ABI_SYMBOLS = (
     "hc_create", "hc_destroy", "hc_last_error", "hc_set_mesh", "hc_set_case", "hc_set_precision",
-    "hc_set_boundary_refs", "hc_set_state", "hc_get_state", "hc_residual", "hc_step", "hc_get_time",
+    "hc_set_boundary_refs", "hc_set_state", "hc_get_state", "hc_residual", "hc_step", "hc_step_steady", "hc_get_time",
     "hc_norms", "hc_get_status", "hc_synchronize", "hc_set_graph_steps", "hc_set_profiling",
     "hc_get_profile", "hc_device_state", "hc_stream_triad", "hc_version",
     "hc_batch_begin", "hc_batch_add_case", "hc_batch_end", "hc_batch_size", "hc_batch_set_boundary_refs",
@@ -79,6 +79,7 @@ def library():
         "hc_get_state": (i, [P, P]),
         "hc_residual": (i, [P, P]),
         "hc_step": (i, [P, i]),
+        "hc_step_steady": (i, [P, C.c_longlong, d, i, P, P]),
         "hc_get_time": (i, [P, P]),
         "hc_norms": (i, [P, P, P]),
         "hc_get_status": (i, [P, P]),
@@ -187,6 +188,13 @@ class Context:
 
     def step(self, nsteps):
         self._check(self._L.hc_step(self._h, int(nsteps)), "hc_step")
+
+    def step_steady(self, max_steps, residual_drop, check_every=50):
+        """Steady-state marching (hc_step_steady): (steps taken, residual drop reached)."""
+        n, drop = C.c_longlong(), C.c_double()
+        self._check(self._L.hc_step_steady(self._h, int(max_steps), float(residual_drop), int(check_every),
+                                           C.byref(n), C.byref(drop)), "hc_step_steady")
+        return n.value, drop.value
 
     def synchronize(self):
         self._check(self._L.hc_synchronize(self._h), "hc_synchronize")

@@ -1,9 +1,10 @@
 // cylinder2D.cpp -- the reference's cylinder driver pattern (cpp/script/euler/cylinder2D.cpp:73-190)
 // on the drop-in API: Euler on an O-grid around a unit-diameter cylinder, slip wall, Riemann far
 // field, periodic theta, Roe flux with MinMod2 MUSCL, classical RK4 with local time stepping.
-//   usage: cylinder2D [nth nr nsteps [mixed [outdir]]]   -- prints the state checksum and timing;
-//   with outdir, writes outdir/result.dat and outdir/mesh.dat in the columns and blocks of the
-//   reference driver (cylinder2D.cpp:33-63, 192-272), for its gnuplot scripts
+//   usage: cylinder2D [nth nr nsteps [mixed [outdir [drop]]]]   -- prints the state checksum and
+//   timing; with outdir (or "-"), writes outdir/result.dat and outdir/mesh.dat in the columns and
+//   blocks of the reference driver (cylinder2D.cpp:33-63, 192-272), for its gnuplot scripts; with
+//   drop, marches to a steady state (SteadyTimeControls, controls.h:31-48) for at most nsteps
 #include <hcamd/hcamd.hpp>
 
 #include <chrono>
@@ -72,7 +73,14 @@ int main(int argc, char** argv)
     policy.precision = mixed ? HC_PRECISION_MIXED : HC_PRECISION_FP64;
 
     const auto t0 = std::chrono::steady_clock::now();
-    integrate(policy, timeControls, rk, hoflux, boundaryConditions, species, mesh, q);
+    if (argc > 6) {
+        const SteadyTimeControls<Real> steady{.nTimesteps = nt, .cfl = 1.0, .residual_drop = std::atof(argv[6]),
+                                              .use_residual_smoothing = false, .smoothing_factor = 1.0};
+        const Real drop = integrate(policy, steady, rk, hoflux, boundaryConditions, species, mesh, q);
+        std::printf("steady: residual drop %.6e\n", drop);
+    } else {
+        integrate(policy, timeControls, rk, hoflux, boundaryConditions, species, mesh, q);
+    }
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 
     double sum = 0.0, pmin = 1e300;
@@ -82,7 +90,7 @@ int main(int argc, char** argv)
     }
     std::printf("cylinder2D %zux%zu steps %zu: checksum %.15e pmin %.6f time %.3f s\n", nth, nr, nt, sum, pmin, s);
 
-    if (argc > 5) {
+    if (argc > 5 && std::string(argv[5]) != "-") {
         // cells in (r, theta) order, theta fastest, a blank line after each ring; the reference
         // driver's coordinates are (theta, z = 0, r) of the cell centre (its r is uniform, the
         // stretched O-grid's is the centre's distance from the axis)

@@ -543,6 +543,15 @@ template <floating_point Real> struct UnsteadyTimeControls {
     Real cfl;
 };
 
+// controls.h:31-48
+template <floating_point Real> struct SteadyTimeControls {
+    std::size_t nTimesteps;        // at most this many steps
+    Real cfl;
+    Real residual_drop;            // stop once the mass residual's L2 norm has dropped by this factor
+    bool use_residual_smoothing;   // not built: must be false
+    Real smoothing_factor;
+};
+
 // execution policies (parallalg/parallalg.h:65-87); the work runs on the GPU whichever is passed
 namespace par {
 namespace execution {
@@ -689,6 +698,37 @@ Real integrate(const Policy policy, const UnsteadyTimeControls<Real>& timeContro
     hc_destroy(ctx);
     if (status) throw std::runtime_error("device raised status bits " + std::to_string(status));
     return Real(t);
+}
+
+/*
+ * Steady-state integrate with SteadyTimeControls (controls.h:31-48, for which the reference has no
+ * integrate at HEAD): marches until the mass residual has dropped by timeControls.residual_drop
+ * (hc_step_steady, checked every 50 steps) or nTimesteps are taken, writes the state back into q0
+ * and returns the drop reached.  Residual smoothing is not built.
+ */
+template <typename Policy, LawType Law, int nDim, floating_point Real, auto Basis, typename SecondOrderFlux,
+          typename... BoundaryConds>
+Real integrate(const Policy policy, const SteadyTimeControls<Real>& timeControls,
+               const ODE::Explicit::RungeKutta<Real>& rungeKutta, const SecondOrderFlux& flux2,
+               const std::tuple<BoundaryConds...> boundaryConds, const Species<Law, Real>& species,
+               const Mesh<nDim, Real>& mesh, SolutionField<VariableSet<Law, nDim, Basis, Real>, nDim>& q0)
+{
+    if (timeControls.use_residual_smoothing) throw std::runtime_error("residual smoothing is not built");
+    double alpha[6][6], beta[6];
+    for (int s = 0; s < 6; ++s) {
+        beta[s] = rungeKutta.beta[s];
+        for (int k = 0; k < 6; ++k) alpha[s][k] = rungeKutta.alpha[s][k];
+    }
+    hc_context* ctx = hcamd_detail::setup<Law, nDim, Basis>(hcamd_detail::hip_of(policy), flux2, boundaryConds, species,
+                                                            mesh, q0, (int)rungeKutta.nstages, alpha, beta,
+                                                            timeControls.cfl);
+    long long steps = 0;
+    double drop = 1.0;
+    hcamd_detail::check(hc_step_steady(ctx, (long long)timeControls.nTimesteps, timeControls.residual_drop, 50, &steps,
+                                       &drop), ctx, "hc_step_steady");
+    hcamd_detail::check(hc_get_state(ctx, reinterpret_cast<double*>(q0.interior.data())), ctx, "hc_get_state");
+    hc_destroy(ctx);
+    return Real(drop);
 }
 
 /*

@@ -933,6 +933,34 @@ int hc_batch_residual(hc_context* ctx, int k, double* res) { return residual(ctx
 int hc_batch_norms(hc_context* ctx, int k, double* l2, double* linf) { return norms(ctx, k, l2, linf); }
 int hc_batch_get_time(hc_context* ctx, int k, double* t) { return get_time(ctx, k, t); }
 
+int hc_step_steady(hc_context* ctx, long long max_steps, double residual_drop, int check_every,
+                   long long* steps_done, double* drop)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (max_steps < 0 || check_every < 1 || !(residual_drop > 0.0) || !steps_done || !drop)
+        return fail(ctx, HC_ERR_ARG, "steady stepping: max_steps >= 0, check_every >= 1, residual_drop > 0 and outputs");
+    if (ctx->cases.size() != 1) return fail(ctx, HC_ERR_STATE, "steady stepping drives a single-case context");
+    *steps_done = 0;
+    *drop = 1.0;
+    double l2[8], linf[8];
+    r = hc_norms(ctx, l2, linf);
+    if (r) return r;
+    const double r0 = l2[2];
+    if (!(r0 > 0.0)) return HC_OK;
+    while (*steps_done < max_steps) {
+        const int n = (int)std::min<long long>(check_every, max_steps - *steps_done);
+        r = hc_step(ctx, n);
+        if (r) return r;
+        *steps_done += n;
+        r = hc_norms(ctx, l2, linf);
+        if (r) return r;
+        *drop = l2[2] / r0;
+        if (*drop <= residual_drop) break;
+    }
+    return HC_OK;
+}
+
 int hc_step(hc_context* ctx, int nsteps)
 {
     int r = ready(ctx);

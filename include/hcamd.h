@@ -128,6 +128,17 @@ int hc_residual(hc_context* ctx, double* res);
  */
 int hc_step(hc_context* ctx, int nsteps);
 
+/*
+ * Steady-state marching: the parameters of SteadyTimeControls (controls.h:31-48; at the
+ * reference's HEAD no integrate() consumes them).  Steps in chunks of check_every (graphs as
+ * hc_step) until the L2 norm of the residual's third variable (mass for Euler, the transported
+ * scalar for advection; see hc_norms) has dropped to residual_drop times its value at the
+ * start, or max_steps are taken.  Single-case contexts.  On return *steps_done holds the steps
+ * taken and *drop the last ratio (1 if the start residual is zero).
+ */
+int hc_step_steady(hc_context* ctx, long long max_steps, double residual_drop, int check_every,
+                   long long* steps_done, double* drop);
+
 /* Sum of the global time steps taken (the reference's t, rungeKutta.h:96,163). */
 int hc_get_time(hc_context* ctx, double* t);
 

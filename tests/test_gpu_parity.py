@@ -171,6 +171,30 @@ def test_gresho_reference_driver_matches_oracle():
     assert abs(t - to) <= 1e-12 * abs(to)
 
 
+def test_steady_marching_stops_at_the_residual_drop():
+    """hc_step_steady on the face-frame Roe cylinder (64 x 32, local dt): it stops at the first
+    check with the mass residual's L2 norm below 1e-2 of the start, its state equals plain stepping
+    for that many steps, and the drop it reports is the oracle's at that step."""
+    import hyperboliconvergence_amd as hca
+    cfg = _cyl_cfg(64, 32, flux=hca.FLUX_ROE_FACEFRAME)
+    with _ctx(cfg) as ctx:
+        n, drop = ctx.step_steady(5000, 1e-2, 100)
+        q = ctx.get_state()
+    assert n % 100 == 0 and 100 <= n < 5000 and drop <= 1e-2
+    with _ctx(cfg) as ctx:
+        ctx.step(n - 100)
+        l2_before = ctx.norms()[0][2]
+        ctx.step(100)
+        assert np.array_equal(ctx.get_state(), q)
+    cells = po.dual(cfg["nodes"])
+    l2_0 = po.norms(cells, _oracle_res(cfg))[0][2]
+    qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], n, cfg["bref"])
+    l2_n = po.norms(cells, po.residual(cfg["case"], cfg["nodes"], qo, cfg["bref"]))[0][2]
+    assert abs(drop - l2_n / l2_0) <= 1e-6 * drop
+    assert l2_before / l2_0 > 1e-2   # the previous check had not converged
+    print(f"steady: {n} steps, drop {drop:.3e}")
+
+
 def test_graph_replay_matches_eager():
     cfg = _cyl_cfg(64, 32)
     with _ctx(cfg) as a:
