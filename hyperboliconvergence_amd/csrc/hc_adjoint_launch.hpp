@@ -4,7 +4,7 @@
 
 #include "hc_adjoint.hpp"
 
-#define HC_FOR_EACH_ADJ(X) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 1, 4)
+#define HC_FOR_EACH_ADJ(X) X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 0, 5) X(1, 0, 6) X(1, 1, 4)
 
 #define HC_DECL_ADJ(L, B, F)                                                             \
     hipError_t hc_tan_L##L##_B##B##_F##F(const hc::AdjArgs& a, hipStream_t st);          \

@@ -216,9 +216,9 @@ int hc_set_tiles(hc_context* ctx, int config);
 int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step);
 
 /*
- * Derivatives (single-case context, FP64; Euler with Rusanov, Roe, SLAU or AUSM+up in the
- * conserved basis, AUSM+up also in the primitive basis).  No counterpart in the reference.
- * Parameters: the state q, minf of the species (AUSM+up / SLAU scaling) and the boundary
+ * Derivatives (single-case context, FP64; Euler with every flux in the conserved basis, AUSM+up
+ * also in the primitive basis).  No counterpart in the reference.
+ * Parameters: the state q, minf of the species (AUSM+up / SLAU / RoeUnprecWS scaling) and the boundary
  * reference states bref ([rows][nvar] as hc_set_boundary_refs).
  *
  * hc_tangent_residual: R(q) of the current state ([N][nvar+1], as hc_residual) and

@@ -75,7 +75,7 @@ template <int LAW, int BASIS, int FLUX> void run_adjoint(AdjArgs& a)
     for (long long c = 0; c < a.N; ++c) K::adj_qedges(a, c);
 }
 
-#define ADJ_KEYS(X) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 1, 4)
+#define ADJ_KEYS(X) X(1, 0, 0) X(1, 0, 1) X(1, 0, 2) X(1, 0, 3) X(1, 0, 4) X(1, 0, 5) X(1, 0, 6) X(1, 1, 4)
 
 bool dispatch(const hc_case_desc* cs, AdjArgs& a, bool tangent)
 {

@@ -51,7 +51,9 @@ def _fd(cs, nodes, q, bref, qd=None, bd=None, md=0.0, h=1e-7):
 
 
 SMOOTH = [(po.FLUX_AUSMPUP, po.LIM_SMOOTHVANALBADA), (po.FLUX_ROE, po.LIM_VANALBADA2),
-          (po.FLUX_SLAU, po.LIM_SMOOTHVANALBADA), (po.FLUX_RUSANOV, po.LIM_NOLIMIT1)]
+          (po.FLUX_SLAU, po.LIM_SMOOTHVANALBADA), (po.FLUX_RUSANOV, po.LIM_NOLIMIT1),
+          (po.FLUX_CENTRAL, po.LIM_SMOOTHVANALBADA), (po.FLUX_ROE_FACEFRAME, po.LIM_SMOOTHVANALBADA),
+          (po.FLUX_ROE_UNPREC_WS, po.LIM_SMOOTHVANALBADA)]
 
 
 @pytest.mark.parametrize("flux,limiter", SMOOTH)

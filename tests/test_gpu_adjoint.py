@@ -37,8 +37,12 @@ def _ctx(cfg):
     return ctx
 
 
-def test_tangent_and_adjoint_residual_match_host_execution():
+@pytest.mark.parametrize("flux", [0, 1, 2, 3, 4, 5, 6])
+def test_tangent_and_adjoint_residual_match_host_execution(flux):
+    import hyperboliconvergence_amd as hca
     cfg = _cfg()
+    cfg["case"].flux = flux
+    cfg["case"].pscale = hca.SCALE_ACOUSTIC if flux == hca.FLUX_SLAU else hca.SCALE_CONVECTIVE
     rng = np.random.default_rng(1)
     qd = rng.standard_normal(cfg["q"].shape)
     bd = rng.standard_normal(cfg["bref"].shape)
