@@ -1089,28 +1089,37 @@ int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps)
 {
     if (!ctx || n <= 0 || reps <= 0 || !gbps) return HC_ERR_ARG;
     HC_HIP(hipSetDevice(ctx->device));
-    double *a = nullptr, *b = nullptr, *c = nullptr;
-    HC_HIP(hipMalloc(&a, n * sizeof(double)));
-    HC_HIP(hipMalloc(&b, n * sizeof(double)));
-    HC_HIP(hipMalloc(&c, n * sizeof(double)));
+    // released on every return path
+    struct Scratch {
+        double *a = nullptr, *b = nullptr, *c = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Scratch()
+        {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            (void)hipFree(a);
+            (void)hipFree(b);
+            (void)hipFree(c);
+        }
+    } w;
+    HC_HIP(hipMalloc(&w.a, n * sizeof(double)));
+    HC_HIP(hipMalloc(&w.b, n * sizeof(double)));
+    HC_HIP(hipMalloc(&w.c, n * sizeof(double)));
     const unsigned grid = 256 * 32;
-    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, ctx->stream, b, 1.0, n);
-    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, ctx->stream, c, 2.0, n);
-    hipLaunchKernelGGL(k_triad, dim3(grid), dim3(256), 0, ctx->stream, a, b, c, 3.0, n);   // warm-up
-    hipEvent_t e0, e1;
-    HC_HIP(hipEventCreate(&e0));
-    HC_HIP(hipEventCreate(&e1));
-    HC_HIP(hipEventRecord(e0, ctx->stream));
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_triad, dim3(grid), dim3(256), 0, ctx->stream, a, b, c, 3.0, n);
-    HC_HIP(hipEventRecord(e1, ctx->stream));
-    HC_HIP(hipEventSynchronize(e1));
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, ctx->stream, w.b, 1.0, n);
+    hipLaunchKernelGGL(k_fill, dim3(grid), dim3(256), 0, ctx->stream, w.c, 2.0, n);
+    hipLaunchKernelGGL(k_triad, dim3(grid), dim3(256), 0, ctx->stream, w.a, w.b, w.c, 3.0, n);   // warm-up
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipEventCreate(&w.e0));
+    HC_HIP(hipEventCreate(&w.e1));
+    HC_HIP(hipEventRecord(w.e0, ctx->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_triad, dim3(grid), dim3(256), 0, ctx->stream, w.a, w.b, w.c, 3.0, n);
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipEventRecord(w.e1, ctx->stream));
+    HC_HIP(hipEventSynchronize(w.e1));
     float ms = 0.f;
-    HC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(a);
-    (void)hipFree(b);
-    (void)hipFree(c);
+    HC_HIP(hipEventElapsedTime(&ms, w.e0, w.e1));
     *gbps = 24.0 * (double)n * reps / (ms * 1e-3) / 1e9;
     return HC_OK;
 }
