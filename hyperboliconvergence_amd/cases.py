@@ -302,6 +302,38 @@ def config_gresho(n=256, flux=FLUX_RUSANOV, limiter=LIM_NOLIMIT3, nsteps=10, cfl
                 qref=qref, nsteps=nsteps)
 
 
+def config_soundwave(n, flux=FLUX_ROE, limiter=LIM_SMOOTHVANALBADA, mach=0.5, amplitude=0.01, nwavelengths=1,
+                     cfl=0.5, crossings=1.0, rk="ssp34"):
+    """The forward-travelling sound wave of cases/euler/oneD/periodic.h:44-83 on an n x 4
+    periodic strip over [0, 1]: background u0 = rho0 = 1, a0 = u0 / M, p0 = rho0 a0^2 / gamma;
+    p = p0 (1 + M A sin(k x)), rho from the isentrope, u from the u - 2a/(gamma-1) invariant of
+    the u + a wave.  Fixed step of cfl h / (u0 + a0) (the update divides by the reference's
+    area / 4 volume, so dt is a quarter of that), run for the given number of domain crossings."""
+    sp = air_species(mach)
+    w = 4
+    nodes = mesh_linspace(n, w, 0.0, 1.0, 0.0, w / n)
+    x = cell_centres(nodes)[..., 0]
+    g = sp.gamma
+    u0, r0 = 1.0, 1.0
+    a0 = u0 / mach
+    p0 = r0 * a0 * a0 / g
+    k = 2.0 * math.pi * nwavelengths
+    p = p0 * (1.0 + mach * amplitude * np.sin(k * x))
+    r = r0 * np.power(p / p0, 1.0 / g)
+    a = np.sqrt(g * p / r)
+    q = np.empty((n, w, 4))
+    q[..., 0] = u0 + 2.0 * (a - a0) / (g - 1.0)
+    q[..., 1] = 0.0
+    q[..., 2] = r
+    q[..., 3] = p
+    dt = cfl * (1.0 / n) / (u0 + a0)
+    nsteps = int(round(crossings / (u0 + a0) / dt))
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, flux, limiter, (BC_PERIODIC,) * 4, rk, dt / 4.0, DT_FIXED, sp,
+                   period_i=(1.0, 0.0), period_j=(0.0, w / n))
+    return dict(name=f"euler-soundwave-{n}", nodes=nodes, q=q, case=cs, bref=np.zeros((bref_size(n, w), 4)),
+                nsteps=nsteps, t_end=nsteps * dt)
+
+
 def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):
     """Isentropic vortex on a periodic n x n grid over [-10,10]^2, M=0.5 diagonal, strength 5;
     SLAU or AUSM+up, VanAlbada2, SSP-RK3, CFL 0.8."""

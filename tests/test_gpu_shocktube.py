@@ -51,6 +51,19 @@ def test_shock_tube_matches_oracle_and_converges(flux):
     assert em <= 1.01 * e400
 
 
+@pytest.mark.parametrize("flux", FLUXES.values(), ids=FLUXES.keys())
+def test_sound_wave_matches_oracle(flux):
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    cfg = cases.config_soundwave(128, flux)
+    if flux == hca.FLUX_SLAU:
+        cfg["case"].pscale = hca.SCALE_ACOUSTIC
+    q, t = _run(cfg)
+    qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+    assert rel_err(q, qo) <= TRAJ_TOL
+    assert abs(t - to) <= 1e-12 * to
+
+
 def test_reference_roe_is_transversally_unstable_and_face_frame_roe_is_not():
     """A-Q2 on the strip: the reference Roe dissipation mixes the global-frame Roe velocity into
     the face frame of the transverse faces, so the round-off level transverse velocity grows

@@ -1,6 +1,7 @@
-"""Shock tubes (script/euler/shocktube.cpp, cases/euler/oneD/shocktubes.h) on the CPU: the exact
-Riemann solver used as the reference solution, the oracle's convergence to it for every Euler
-flux, and the kernels' host execution against the oracle.
+"""1-D cases on the CPU: the shock tubes of script/euler/shocktube.cpp / cases/euler/oneD/shocktubes.h
+(the exact Riemann solver used as the reference solution, the oracle's convergence to it for
+every Euler flux, the kernels' host execution against the oracle) and the sound wave of
+cases/euler/oneD/periodic.h (second-order convergence).
 
 The reference's shock tube is 1-D; here it runs on an n x 4 strip of square cells, periodic
 across the strip (cases.config_shocktube), which leaves the flow one-dimensional.
@@ -70,6 +71,27 @@ def test_oracle_converges_to_the_exact_solution(flux):
     print(flux, errs)
     assert np.all(errs[1:] <= errs[:-1] / 1.6)
     assert errs[-1, 0] <= 2e-3
+
+
+@pytest.mark.parametrize("flux", FLUXES.values(), ids=FLUXES.keys())
+def test_sound_wave_converges_at_second_order(flux):
+    """The forward-travelling sound wave of cases/euler/oneD/periodic.h (M = 0.5, amplitude 0.01,
+    one domain crossing): the LSQ-MUSCL scheme with the smooth van Albada limiter is second order
+    for every flux.  Self-convergence: the pressure on n cells against the 2n solution averaged
+    pairwise, for n = 32, 64, 128; observed orders 1.9 and 2.0."""
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    sols = {}
+    for n in (32, 64, 128, 256):
+        cfg = cases.config_soundwave(n, flux)
+        if flux == hca.FLUX_SLAU:
+            cfg["case"].pscale = hca.SCALE_ACOUSTIC
+        q, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+        sols[n] = q[:, 0, 3]
+    errs = np.array([np.abs(sols[n] - sols[2 * n].reshape(n, 2).mean(axis=1)).mean() for n in (32, 64, 128)])
+    orders = np.log2(errs[:-1] / errs[1:])
+    print(flux, errs, orders)
+    assert np.all(orders >= 1.8)
 
 
 def test_host_execution_of_the_kernels_is_bit_identical_to_the_oracle():
