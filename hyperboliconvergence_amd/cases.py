@@ -334,6 +334,49 @@ def config_soundwave(n, flux=FLUX_ROE, limiter=LIM_SMOOTHVANALBADA, mach=0.5, am
                 nsteps=nsteps, t_end=nsteps * dt)
 
 
+def mesh_channelbump(d=2.0, l=3.0, w=2.0, h=0.1, nd=96, nl=48, nw=48):
+    """make_channelbump_mesh (mesh/generate/twoD.h:49-130): a channel of height d with a cosine
+    bump of height h and width w on its lower wall between two flat sections of length l;
+    uniform cells in the flat sections, and over the bump nodes evenly spaced from the bump
+    y = h/2 (1 + cos(2 pi x / w)) to the upper wall.  Nodes [2 nl + nw + 1][nd + 1][2]."""
+    nx, ny = 2 * nl + nw, nd
+    nodes = np.empty((nx + 1, ny + 1, 2))
+    xl0, xw0, xr0 = -(l + 0.5 * w), -0.5 * w, 0.5 * w
+    dxl, dyl = l / nl, d / ny
+    for i in range(nl + 1):
+        for j in range(ny + 1):
+            nodes[i, j] = (xl0 + i * dxl, j * dyl)
+            nodes[nl + nw + i, j] = (xr0 + i * dxl, j * dyl)
+    dxw = w / nw
+    for i in range(nw):
+        ii = i + nl + 1
+        x = xw0 + (i + 1) * dxw
+        yl = 0.5 * h * (1.0 + math.cos(2.0 * math.pi * x / w))
+        dpy = (d - yl) * (1.0 / ny)
+        for j in range(ny):
+            nodes[ii, j] = (x, yl + j * dpy)
+        nodes[ii, ny] = (x, d)
+    return nodes
+
+
+def config_channelbump(nd=96, nl=48, nw=48, flux=FLUX_RUSANOV, limiter=None, nsteps=6500, cfl=0.4, mach=0.4,
+                       rk="ssp11"):
+    """The reference's channel-bump driver (script/euler/channelBump2D.cpp:58-140): d = 2, l = 3,
+    w = 2, h = 0.1, free stream M = 0.4 (p = rho = 1) along x, fixed states at inflow and outflow
+    (ids 0, 1), slip walls below and above (ids 2, 3), Rusanov + NoLimit1, forward Euler (ssp11),
+    global dt at CFL 0.4, 6500 steps."""
+    from . import BC_FIXED, LIM_NOLIMIT1
+    nodes = mesh_channelbump(nd=nd, nl=nl, nw=nw)
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    speed = mach * math.sqrt(1.4 * 1.0 / 1.0)
+    qref = np.array([speed, 0.0, 1.0, 1.0])
+    q = np.broadcast_to(qref, (ni, nj, 4)).copy()
+    cs = make_case(LAW_EULER, BASIS_PRIMITIVE, flux, LIM_NOLIMIT1 if limiter is None else limiter,
+                   (BC_FIXED, BC_FIXED, BC_WALL, BC_WALL), rk, cfl, DT_GLOBAL, air_species(mach))
+    return dict(name=f"euler-channelbump-{ni}x{nj}", nodes=nodes, q=q, case=cs,
+                bref=np.broadcast_to(qref, (bref_size(ni, nj), 4)).copy(), qref=qref, nsteps=nsteps)
+
+
 def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):
     """Isentropic vortex on a periodic n x n grid over [-10,10]^2, M=0.5 diagonal, strength 5;
     SLAU or AUSM+up, VanAlbada2, SSP-RK3, CFL 0.8."""

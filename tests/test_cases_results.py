@@ -1,5 +1,6 @@
 """The case layer beyond the benchmark configurations: the Gresho vortex of script/euler/gresho2D.cpp
-(its profile pinned by radial equilibrium and continuity, its low-Mach behaviour on the oracle)
+(its profile pinned by radial equilibrium and continuity, its low-Mach behaviour on the oracle),
+the channel with a bump of script/euler/channelBump2D.cpp (mesh closed forms, steady mass flux)
 and the result writers with the reference drivers' column formats."""
 import math
 import re
@@ -55,6 +56,36 @@ def test_gresho_low_mach_dissipation_of_the_fluxes():
     print("kinetic energy kept:", unprec, ff, rus)
     assert unprec >= 0.998 and ff >= 0.98 and rus <= 0.95
     assert unprec > ff > rus
+
+
+def test_channel_bump_mesh_and_steady_flow():
+    """make_channelbump_mesh: flat walls y = 0 and y = d outside the bump, the cosine bump of height
+    h on the lower wall (top at x = 0), uniform spacing along each node line; the reference driver's
+    case (Rusanov, first order, forward Euler, 6500 steps at CFL 0.4, M = 0.4) reaches a flow whose
+    mass flux is the same through every node line to 0.3 % and whose pressure dips over the bump
+    (subsonic acceleration), on the oracle."""
+    from hyperboliconvergence_amd import cases
+    n = cases.mesh_channelbump()
+    assert n.shape == (145, 97, 2)
+    x = n[:, 0, 0]
+    assert x[0] == -4.0 and x[-1] == 4.0 and np.all(np.diff(x) > 0)
+    assert np.all(n[:, -1, 1] == 2.0)
+    bump = n[:, 0, 1]
+    assert np.all(bump[np.abs(x) >= 1.0] == 0.0) and bump[72] == 0.1 and np.all(bump <= 0.1)
+    assert np.allclose(bump, np.where(np.abs(x) < 1.0, 0.05 * (1 + np.cos(np.pi * x)), 0.0), atol=1e-15)
+    assert np.allclose(np.diff(n[:, :, 1], axis=1), (2.0 - bump[:, None]) / 96, rtol=1e-12)
+    cfg = cases.config_channelbump()
+    q, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["nsteps"], cfg["bref"])
+    assert np.all(np.isfinite(q))
+    flows = []
+    for i in (1, 30, 60, 72, 84, 110, 143):
+        dy = n[i, 1:, 1] - n[i, :-1, 1]
+        qq = 0.5 * (q[i - 1] + q[i]) if i < q.shape[0] else q[-1]
+        flows.append(float((qq[:, 2] * qq[:, 0] * dy).sum()))
+    flows = np.array(flows)
+    assert np.ptp(flows) <= 3e-3 * flows.mean()
+    p = q[..., 3]
+    assert p[60:85, 0].min() < 0.95 < 1.0 < p[:5, 0].max()
 
 
 def _lines(path):
