@@ -412,7 +412,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+              "(batches would shard by case, a large grid by slabs of rows with a 2-cell halo exchange)", file=sys.stderr)
         sys.exit(2)
     if "RANK" in os.environ and int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only", file=sys.stderr)

@@ -53,7 +53,7 @@ def test_example_driver_builds():
 
 @pytest.mark.gpu
 def test_example_driver_runs_and_matches_oracle(tmp_path):
-    """The driver's checksum against the oracle, and its result.dat / mesh.dat (written through the
+    """The example driver's checksum against the oracle, and its result.dat / mesh.dat (written through the
     drop-in set2State) against the Python writers on the oracle's final state."""
     from hyperboliconvergence_amd import cases, results
     exe = os.path.join(CPP, "examples", "cylinder2D")
@@ -75,7 +75,7 @@ def test_example_driver_runs_and_matches_oracle(tmp_path):
         assert a.shape == b.shape
         # 8 significant digits, GPU vs oracle within 1e-10, relative columns of small numbers
         assert np.allclose(a, b, rtol=1e-7, atol=1e-9), name
-    # steady marching through the drop-in's SteadyTimeControls integrate: the driver's reference Roe
+    # steady marching through the drop-in's SteadyTimeControls integrate: the example's reference Roe
     # takes the mass residual below 5e-2 of its start within 200 steps (the oracle: 4.7e-2 at
     # step 200), well before the frame-mixing divergence near step 400 on this grid
     out = subprocess.run([exe, "64", "32", "2000", "0", "-", "5e-2"], capture_output=True, text=True, check=True,

@@ -11,7 +11,7 @@ exercise: indexing, tiling, batching and graph replay at their real shapes).
   after 20 steps of the vortex);
 * every tile configuration gives the same bits on the 512 x 256 cylinder and the 96-case batch;
 * graph replay equals eager stepping on the cylinder;
-* the mixed path stays within the north-star bound of FP64 on the 96-case batch;
+* the mixed path stays within 1e-6 of FP64 on the 96-case batch;
 * on the 4096^2 vortex the mixed path departs from FP64 no further than FP64 does from itself
   when its initial state is perturbed by an FP32-sized amount.
 """

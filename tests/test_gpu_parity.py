@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C-ABI against the CPU oracle on identical inputs.
 
-Tolerances (north_star): FP64 residual <= 1e-12 relative, FP64 state after the
+Tolerances: FP64 residual <= 1e-12 relative, FP64 state after the
 stated steps <= 1e-10 relative; mixed FP32-face / FP64-state <= 1e-6 relative.
 Relative errors are normwise: max|a - b| / max|b|.
 """
