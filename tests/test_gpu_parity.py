@@ -97,7 +97,7 @@ def test_vortex_residual_matches_oracle(flux):
 
 
 def test_c1_full_trajectory_matches_oracle():
-    """Config 1 at its stated size: 128x128, 200 SSP-RK3 steps (VanLeer2 verbatim)."""
+    """The scalar advection workload at its full size: 128x128, 200 SSP-RK3 steps (VanLeer2 verbatim)."""
     cfg = _scalar_cfg(128)
     with _ctx(cfg) as ctx:
         ctx.step(cfg["nsteps"])
