@@ -50,13 +50,13 @@ def make_cfg(name, steps):
     from hyperboliconvergence_amd import cases
     import hyperboliconvergence_amd as hca
     if name == "c1":
-        cfg = cases.config_c1(128)
+        cfg = cases.config_scalar_advection(128)
         g, window = 0.0, None
     elif name == "c3":
-        cfg = cases.config_c3(4096, hca.FLUX_AUSMPUP)
+        cfg = cases.config_isentropic_vortex(4096, hca.FLUX_AUSMPUP)
         g, window = 0.0, None
     else:
-        cfg = cases.config_c2()
+        cfg = cases.config_cylinder_ogrid()
         g, window = 16.0, 300
     cfg["geom_bytes"] = g
     cfg["bench_steps"] = window or cfg["nsteps"]

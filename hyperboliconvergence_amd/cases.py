@@ -191,7 +191,7 @@ def isentropic_vortex(x, y, t=0.0, beta=5.0, mach=0.5, gamma=GAMMA, x0=0.0, y0=0
 
 # ---------------- benchmark configurations ----------------
 
-def config_c1(n=128):
+def config_scalar_advection(n=128):
     """Scalar advection on a periodic n x n grid over [0,1]^2, velocity (1, 0.5), Gaussian
     bump, Rusanov (= upwind for linear advection), VanLeer2, SSP-RK3, CFL 0.5, FP64."""
     nodes = mesh_linspace(n, n, 0.0, 1.0, 0.0, 1.0)
@@ -206,7 +206,7 @@ def config_c1(n=128):
                 bref=np.zeros((bref_size(n, n), 3)), nsteps=200)
 
 
-def config_c2(nth=512, nr=256, mach=0.3, nsteps=2000):
+def config_cylinder_ogrid(nth=512, nr=256, mach=0.3, nsteps=2000):
     """Euler O-grid around a unit-diameter cylinder, 512 (theta) x 256 (r) cells, r from 0.5 to 20
     with first spacing 1e-3, freestream M=0.3 (rho=1, p=1/gamma), slip wall (id 0), Riemann far field
     (id 1), periodic theta (ids 2, 3); Roe + Harten fix, MinMod2, LSQ, RK4 with local dt at CFL 1."""
@@ -377,7 +377,7 @@ def config_channelbump(nd=96, nl=48, nw=48, flux=FLUX_RUSANOV, limiter=None, nst
                 bref=np.broadcast_to(qref, (bref_size(ni, nj), 4)).copy(), qref=qref, nsteps=nsteps)
 
 
-def config_c3(n=4096, flux=FLUX_SLAU, nsteps=500):
+def config_isentropic_vortex(n=4096, flux=FLUX_SLAU, nsteps=500):
     """Isentropic vortex on a periodic n x n grid over [-10,10]^2, M=0.5 diagonal, strength 5;
     SLAU or AUSM+up, VanAlbada2, SSP-RK3, CFL 0.8."""
     nodes = mesh_linspace(n, n, -10.0, 10.0, -10.0, 10.0)
@@ -448,7 +448,7 @@ def batch_case_table(ncases=96, seed=20261202):
 
 
 def config_cylinder(nth, nr, mach, alpha_deg=0.0, flux=FLUX_ROE, nsteps=1000):
-    """One case of the cylinder batch: the O-grid of config_c2 at (n_theta, n_r), freestream
+    """One case of the cylinder batch: the O-grid of config_cylinder_ogrid at (n_theta, n_r), freestream
     (u, v) = M (cos a, sin a), rho = 1, p = 1/gamma; Roe, MinMod2, LSQ, RK4, local dt at CFL 1."""
     nodes = mesh_ogrid(nr, nth)
     a = math.radians(alpha_deg)

@@ -94,7 +94,7 @@ def _lines(path):
 
 def test_cylinder_and_mesh_writers(tmp_path):
     from hyperboliconvergence_amd import cases, results
-    cfg = cases.config_c2(nth=16, nr=8)
+    cfg = cases.config_cylinder_ogrid(nth=16, nr=8)
     qref = cfg["q"][0, 0]
     results.write_cylinder_result(tmp_path / "r.dat", cfg["nodes"], cfg["q"], qref)
     lines = _lines(tmp_path / "r.dat")
@@ -132,7 +132,7 @@ def test_gresho_shocktube_and_scalar_writers(tmp_path):
     # at rest: u/a = u = 0; entropy a^2 / (gamma rho^(gamma-1)) = p / rho^gamma
     assert np.allclose(s, np.stack([rho, 0 * rho, 0 * rho, p, p / rho ** 1.4], axis=1), rtol=1e-8, atol=1e-300)
 
-    c = cases.config_c1(6)
+    c = cases.config_scalar_advection(6)
     results.write_scalar_result(tmp_path / "c.dat", c["nodes"], c["q"])
     lines = _lines(tmp_path / "c.dat")
     assert lines.count("") == 6 + 1

@@ -62,7 +62,7 @@ def test_example_driver_runs_and_matches_oracle(tmp_path):
                          timeout=120).stdout
     checksum = float(out.split("checksum")[1].split()[0])
     # the C++ O-grid generator uses the same formula as cases.mesh_ogrid
-    cfg = cases.config_c2(nth=64, nr=32)
+    cfg = cases.config_cylinder_ogrid(nth=64, nr=32)
     qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 50, cfg["bref"])
     assert abs(checksum - qo.sum()) <= 1e-9 * abs(qo.sum())
     results.write_cylinder_result(tmp_path / "py_result.dat", cfg["nodes"], qo, cfg["q"][0, 0])

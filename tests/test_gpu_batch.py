@@ -138,7 +138,7 @@ def test_batch_global_time_step_is_per_case():
     """Global dt (SSP-RK3): each case reduces its own max(lambda/V) and keeps its own time."""
     from hyperboliconvergence_amd import cases
     import hyperboliconvergence_amd as hca
-    cfgs = [cases.config_c3(n, flux=hca.FLUX_AUSMPUP) for n in (24, 40, 33)]
+    cfgs = [cases.config_isentropic_vortex(n, flux=hca.FLUX_AUSMPUP) for n in (24, 40, 33)]
     with _batch(cfgs) as b:
         b.step(4)
         qb = [b.get_case_state(k) for k in range(3)]

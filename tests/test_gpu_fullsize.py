@@ -52,7 +52,7 @@ def _conservation_error(c0, c1):
 def test_vortex_4096_conserves_mass_momentum_energy(precision):
     import hyperboliconvergence_amd as hca
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c3(4096, hca.FLUX_AUSMPUP)
+    cfg = cases.config_isentropic_vortex(4096, hca.FLUX_AUSMPUP)
     with _ctx(cfg, precision, graph_steps=10) as ctx:
         r = ctx.residual()[..., :4].reshape(-1, 4)
         rsum = np.abs(r.sum(axis=0)) / np.abs(r).sum(axis=0)
@@ -78,7 +78,7 @@ def test_vortex_4096_mixed_error_is_the_cases_sensitivity_to_fp32_sized_perturba
     supersonic ring of this grid, smaller ones do not."""
     import hyperboliconvergence_amd as hca
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c3(4096, hca.FLUX_AUSMPUP)
+    cfg = cases.config_isentropic_vortex(4096, hca.FLUX_AUSMPUP)
     sign = np.random.default_rng(11).choice([-1.0, 1.0], size=cfg["q"].shape)
 
     def run(q, precision=0):
@@ -102,7 +102,7 @@ def test_scalar_128_conserves_the_scalar():
     # FP64 only: with the reference's VanLeer2 the scalar grows without bound and the mixed
     # path's FP32 faces leave the finite numbers
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c1(128)
+    cfg = cases.config_scalar_advection(128)
     with _ctx(cfg, 0) as ctx:
         ctx.step(20)
         q = ctx.get_state()
@@ -115,7 +115,7 @@ def test_scalar_128_conserves_the_scalar():
 def test_cylinder_512x256_every_tile_configuration_and_graph_replay():
     import hyperboliconvergence_amd as hca
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c2()
+    cfg = cases.config_cylinder_ogrid()
     out = []
     for tiles in range(hca.NTILECFG):
         with _ctx(cfg, 0, tiles) as ctx:

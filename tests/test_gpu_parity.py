@@ -30,7 +30,7 @@ def _ctx(cfg, precision=0):
 
 def _scalar_cfg(n=128, limiter=None):
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c1(n)
+    cfg = cases.config_scalar_advection(n)
     if limiter is not None:
         cfg["case"].limiter = limiter
     return cfg
@@ -39,7 +39,7 @@ def _scalar_cfg(n=128, limiter=None):
 def _cyl_cfg(nth=64, nr=32, flux=None, basis=None, perturb=0.0):
     from hyperboliconvergence_amd import cases
     import hyperboliconvergence_amd as hca
-    cfg = cases.config_c2(nth=nth, nr=nr)
+    cfg = cases.config_cylinder_ogrid(nth=nth, nr=nr)
     if flux is not None:
         cfg["case"].flux = flux
         cfg["case"].pscale = hca.SCALE_ACOUSTIC if flux == hca.FLUX_SLAU else hca.SCALE_CONVECTIVE
@@ -56,7 +56,7 @@ def _cyl_cfg(nth=64, nr=32, flux=None, basis=None, perturb=0.0):
 def _vortex_cfg(n=64, flux=None):
     from hyperboliconvergence_amd import cases
     import hyperboliconvergence_amd as hca
-    return cases.config_c3(n, flux=hca.FLUX_SLAU if flux is None else flux)
+    return cases.config_isentropic_vortex(n, flux=hca.FLUX_SLAU if flux is None else flux)
 
 
 def _oracle_res(cfg):

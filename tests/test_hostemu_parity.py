@@ -89,7 +89,7 @@ def test_rk_tables_bit_identical(rk, dt):
 
 def test_scalar_c1_trajectory_bit_identical():
     from hyperboliconvergence_amd import cases
-    cfg = cases.config_c1(40)
+    cfg = cases.config_scalar_advection(40)
     qe, te, _ = pyemu.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], 30)
     qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 30, cfg["bref"])
     assert np.array_equal(qe, qo) and te == to
@@ -99,7 +99,7 @@ def test_vortex_periodic_small_grid_smaller_than_tile():
     """nj smaller than the 32-wide tile and periodic in both directions (modular halo wrap)."""
     from hyperboliconvergence_amd import cases
     import hyperboliconvergence_amd as hca
-    cfg = cases.config_c3(12, hca.FLUX_SLAU)
+    cfg = cases.config_isentropic_vortex(12, hca.FLUX_SLAU)
     qe, _, _ = pyemu.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], 4)
     qo, _, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 4, cfg["bref"])
     assert np.array_equal(qe, qo, equal_nan=True)
