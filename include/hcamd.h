@@ -83,7 +83,8 @@ typedef struct {
 
 typedef struct hc_context hc_context;
 
-/* Create / destroy a context on a HIP device (owns a stream and device buffers). */
+/* Create / destroy a context on a HIP device (owns a stream and device buffers); destroying a
+   null context does nothing.  Every other entry point returns HC_ERR_ARG for a null context. */
 int hc_create(hc_context** ctx, int device);
 int hc_destroy(hc_context* ctx);
 const char* hc_last_error(const hc_context* ctx);

@@ -4,6 +4,7 @@ import ctypes as C
 import os
 import re
 import subprocess
+import sys
 
 import pytest
 
@@ -75,3 +76,38 @@ def test_context_requires_built_library_no_fallback(monkeypatch, tmp_path):
     monkeypatch.setattr(hca, "_LIB", None)
     with pytest.raises(hca.HCError):
         hca.Context(0)
+
+
+def test_every_entry_point_rejects_a_null_context():
+    """The C ABI's error contract without a GPU: a null context handle returns HC_ERR_ARG from
+    every entry point that takes one, hc_destroy(NULL) is a no-op (run in a child process, so a
+    crash would fail the test instead of ending the run)."""
+    code = r'''
+import ctypes as C, sys
+sys.path.insert(0, %r)
+import hyperboliconvergence_amd as hca
+L = hca.library()
+bad = []
+for name in hca.C_ABI_SYMBOLS:
+    f = getattr(L, name)
+    args = f.argtypes or []
+    if name in ("hc_create", "hc_version") or not args or args[0] is not C.c_void_p:
+        continue
+    vals = []
+    for t in args:
+        vals.append(None if t is C.c_void_p else t(0) if t is not C.c_double else 0.0)
+    vals[0] = None
+    rc = f(*vals)
+    if name == "hc_last_error":
+        continue
+    if name == "hc_destroy":   # like free(NULL): nothing to release
+        if rc != 0:
+            bad.append((name, rc))
+        continue
+    if rc != -1:
+        bad.append((name, rc))
+print("BAD", bad)
+sys.exit(1 if bad else 0)
+''' % os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
