@@ -84,7 +84,8 @@ typedef struct {
 typedef struct hc_context hc_context;
 
 /* Create / destroy a context on a HIP device (owns a stream and device buffers); destroying a
-   null context does nothing.  Every other entry point returns HC_ERR_ARG for a null context. */
+   null context does nothing.  Every other entry point that takes a context returns HC_ERR_ARG
+   for a null one (hc_last_error returns the message "null context"). */
 int hc_create(hc_context** ctx, int device);
 int hc_destroy(hc_context* ctx);
 const char* hc_last_error(const hc_context* ctx);
