@@ -212,6 +212,39 @@ def test_channel_bump_reference_driver_matches_oracle():
     assert abs(t - to) <= 1e-12 * abs(to)
 
 
+def test_invalid_case_descriptors_and_arguments_are_rejected():
+    """hc_set_case validates the descriptor (flux, limiter, law / flux and law / BC combinations,
+    paired periodic sides, stage count, CFL) and the context stays usable after a rejection."""
+    import hyperboliconvergence_amd as hca
+    cfg = _cyl_cfg(16, 8)
+
+    def bad(**kw):
+        cs = type(cfg["case"]).from_buffer_copy(cfg["case"])
+        for k, v in kw.items():
+            if k == "bc":
+                for i in range(4):
+                    cs.bc[i] = v[i]
+            else:
+                setattr(cs, k, v)
+        return cs
+
+    with _ctx(cfg) as ctx:
+        for cs in (bad(flux=7), bad(flux=-1), bad(limiter=10), bad(nstages=0), bad(nstages=7), bad(cfl=0.0),
+                   bad(bc=(hca.BC_PERIODIC, hca.BC_RIEMANN, hca.BC_PERIODIC, hca.BC_PERIODIC)),
+                   bad(law=hca.LAW_SCALAR, basis=hca.BASIS_CONSERVED, flux=hca.FLUX_ROE)):
+            with pytest.raises(hca.HCError):
+                ctx.set_case(cs)
+        with pytest.raises(hca.HCError):
+            ctx.step(-1)
+        with pytest.raises(hca.HCError):
+            ctx.step_steady(10, 0.0)
+        ctx.set_case(cfg["case"])
+        ctx.set_boundary_refs(cfg["bref"])
+        ctx.set_state(cfg["q"])
+        ctx.step(3)
+        assert ctx.status() == 0
+
+
 def test_graph_replay_matches_eager():
     cfg = _cyl_cfg(64, 32)
     with _ctx(cfg) as a:
