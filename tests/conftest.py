@@ -9,6 +9,11 @@ for sub in ("", "oracle", os.path.join("tests", "hostemu")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# the oracle runs its loops under OpenMP with one thread per CPU by default; parallel test workers
+# (pytest-xdist) would each start a full team and oversubscribe the CPUs many times over
+if os.environ.get("PYTEST_XDIST_WORKER") and "OMP_NUM_THREADS" not in os.environ:
+    os.environ["OMP_NUM_THREADS"] = "1"
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built libhcamd.so")
