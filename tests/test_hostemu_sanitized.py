@@ -1,0 +1,107 @@
+"""The product's stage, tangent and adjoint kernels (host emulation: the same source files executed
+by one host thread per work-group lane set) built as a stand-alone program under AddressSanitizer
+and UndefinedBehaviorSanitizer, run on ragged grids smaller and larger than the tiles, for every
+tile configuration, flux, boundary condition kind, time-step mode and precision: every buffer
+access of the kernels' index arithmetic stays inside its buffer, and the sanitized build gives the
+same bits as the ordinary one."""
+import ctypes as C
+import os
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyemu
+import pyoracle as po
+from test_adjoint_cpu import _ogrid_case, _vortex_case, SMOOTH
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "hostemu")
+DRIVER = os.path.join(HERE, "asan_driver")
+NTILE = 3
+
+
+@pytest.fixture(scope="module")
+def driver():
+    # make rebuilds only when a kernel source changed (a full build takes a few minutes)
+    r = subprocess.run(["make", "-s", "asan_driver"], cwd=HERE, capture_output=True, text=True, timeout=900)
+    if r.returncode != 0 and "sanitize" in r.stderr and "cannot find" in r.stderr:
+        pytest.skip("this compiler has no sanitizer runtime")
+    assert r.returncode == 0, r.stderr[-2000:]
+    return DRIVER
+
+
+def _run(driver, tmp_path, cs, nodes, q, bref, precision=0, nsteps=3, tiles=0, adjoint=None):
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    nv = 4 if cs.law == 1 else 3
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(struct.pack("8i", ni, nj, precision, nsteps, tiles, adjoint is not None, nv, C.sizeof(cs)))
+        f.write(bytes(cs))
+        for a in (nodes, q, bref) + (adjoint or ()):
+            f.write(np.ascontiguousarray(a, float).tobytes())
+    r = subprocess.run([driver, str(fin), str(fout)], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
+                                UBSAN_OPTIONS="print_stacktrace=1"))
+    assert r.returncode == 0, f"rc {r.returncode}\n{r.stderr[-4000:]}"
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+    out = np.fromfile(fout, dtype=np.float64)
+    N, rows = ni * nj, bref.shape[0]
+    sizes = [N * (nv + 1), N * nv, 2]
+    if adjoint is not None:
+        sizes += [N * (nv + 1), N * nv, N * nv, rows * nv, 1]
+    assert out.size == sum(sizes)
+    parts, k = [], 0
+    for s in sizes:
+        parts.append(out[k:k + s])
+        k += s
+    return parts
+
+
+def _check_forward(driver, tmp_path, cfg, precision, tiles, nsteps=3):
+    cs, nodes, q, bref = cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"]
+    res, qs, tail = _run(driver, tmp_path, cs, nodes, q, bref, precision, nsteps, tiles)
+    pyemu.set_tile_config(tiles)
+    try:
+        r_ref, _ = pyemu.residual(cs, nodes, q, bref, precision)
+        q_ref, t_ref, st_ref = pyemu.integrate(cs, nodes, q, bref, nsteps, precision)
+    finally:
+        pyemu.set_tile_config(0)
+    assert np.array_equal(res, r_ref.ravel(), equal_nan=True)
+    assert np.array_equal(qs, q_ref.ravel(), equal_nan=True)
+    assert tail[0] == t_ref and tail[1] == st_ref
+
+
+def _cases():
+    from hyperboliconvergence_amd import cases
+    out = []
+    for flux in range(7):
+        out.append((f"cylinder-20x12-flux{flux}", lambda f=flux: cases.config_cylinder(20, 12, 0.5, 10.0, flux=f)))
+    out.append(("scalar-10x10", lambda: cases.config_scalar_advection(10)))
+    out.append(("vortex-12x12-ausmpup", lambda: cases.config_isentropic_vortex(12, cases.FLUX_AUSMPUP)))
+    out.append(("shocktube-9x4-roe", lambda: cases.config_shocktube(9, cases.FLUX_ROE)))
+    out.append(("channelbump-20x4", lambda: cases.config_channelbump(nd=4, nl=8, nw=8, nsteps=3)))
+    return out
+
+
+@pytest.mark.parametrize("name,make", _cases(), ids=[c[0] for c in _cases()])
+def test_forward_kernels_in_bounds_and_bit_identical(driver, tmp_path, name, make):
+    cfg = make()
+    for tiles in range(NTILE):
+        for precision in (0, 1):
+            if name.startswith("scalar") and precision == 1:
+                continue    # the reference VanLeer2 scalar case grows without bound in FP32 faces
+            _check_forward(driver, tmp_path, cfg, precision, tiles)
+
+
+@pytest.mark.parametrize("flux,limiter", SMOOTH)
+@pytest.mark.parametrize("grid", ["ogrid", "vortex"])
+def test_tangent_and_adjoint_kernels_in_bounds_and_bit_identical(driver, tmp_path, flux, limiter, grid):
+    cs, nodes, q, bref, rng = (_ogrid_case if grid == "ogrid" else _vortex_case)(flux, limiter)
+    qd = rng.standard_normal(q.shape)
+    ybar = rng.standard_normal(q.shape)
+    *_, rt, rd, qbar, bbar, mb = _run(driver, tmp_path, cs, nodes, q, bref, nsteps=1, adjoint=(qd, ybar))
+    rt_ref, rd_ref = pyemu.tangent_residual(cs, nodes, q, bref, qd)
+    qbar_ref, bbar_ref, mb_ref = pyemu.adjoint_residual(cs, nodes, q, bref, ybar)
+    assert np.array_equal(rt, rt_ref.ravel()) and np.array_equal(rd, rd_ref.ravel())
+    assert np.array_equal(qbar, qbar_ref.ravel()) and np.array_equal(bbar, bbar_ref.ravel()) and mb[0] == mb_ref
