@@ -12,15 +12,18 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CPP = os.path.join(ROOT, "hyperboliconvergence_amd", "cpp")
 
 
-def _build_points():
-    exe = os.path.join(ROOT, "tests", "cpp", "dropin_points")
-    subprocess.run(["g++", "-std=c++20", "-O2", "-ffp-contract=off", "-I", CPP, "-I", os.path.join(ROOT, "include"),
+def _build_points(sanitized=False):
+    # the sanitized build checks the functors' host code with AddressSanitizer + UBSan
+    exe = os.path.join(ROOT, "tests", "cpp", "dropin_points" + ("_asan" if sanitized else ""))
+    flags = (["-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=all"] if sanitized else ["-O2"])
+    subprocess.run(["g++", "-std=c++20", *flags, "-ffp-contract=off", "-I", CPP, "-I", os.path.join(ROOT, "include"),
                     os.path.join(ROOT, "tests", "cpp", "dropin_points.cpp"), "-o", exe], check=True)
     return exe
 
 
-def test_dropin_point_functors_bit_identical_to_oracle():
-    exe = _build_points()
+@pytest.mark.parametrize("sanitized", [False, True], ids=["plain", "asan_ubsan"])
+def test_dropin_point_functors_bit_identical_to_oracle(sanitized):
+    exe = _build_points(sanitized)
     rng = np.random.default_rng(21)
     lines, cases = [], []
     for k in range(70):
@@ -34,7 +37,10 @@ def test_dropin_point_functors_bit_identical_to_oracle():
         lines.append(" ".join(repr(float(x)) for x in [fid, lid, *n, area, *ql, *qr]).replace(".0 ", " ", 0))
         cases.append((fid, lid, n, area, ql, qr))
     text = "\n".join(f"{c[0]} {c[1]} " + " ".join(repr(float(x)) for x in [*c[2], c[3], *c[4], *c[5]]) for c in cases)
-    out = subprocess.run([exe], input=text, capture_output=True, text=True, check=True).stdout.split("\n")
+    r = subprocess.run([exe], input=text, capture_output=True, text=True, check=True,
+                       env=dict(os.environ, ASAN_OPTIONS="detect_leaks=0"))
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-4000:]
+    out = r.stdout.split("\n")
     for k, (fid, lid, n, area, ql, qr) in enumerate(cases):
         got = np.array(list(map(float, out[3 * k].split())))
         cs = po.make_case(law=po.LAW_EULER, flux=fid, minf=0.3)
