@@ -10,7 +10,7 @@ Harten fix, MinMod2, LSQ gradients, classical RK4 with local time stepping at CF
 freestream M = 0.3 (synthetic initial condition: uniform freestream).  One "step" is
 one RK4 time step (4 stages) over all cells.
 
-The reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:84-91): on this O-grid
+The reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:40-54): on this O-grid
 its state leaves the finite numbers after about 380 steps, so the default timed window is the
 first 300 steps, where the state is finite; the face-frame Roe variant, which stays finite,
 is timed over the full 2000 steps beside it.  The 4096^2 vortex workload uses AUSM+up: the
@@ -248,7 +248,7 @@ def cpu_baseline_batch(cfgs, budget_s=12.0):
 
 def main_batch(a):
     """96-case cylinder batch.  The reference Roe flux leaves the finite numbers within 50-200 steps on
-    these cases (frame mixing of its dissipation, base/fluxes/roeFlux.ipp:84-91, fastest at high
+    these cases (frame mixing of its dissipation, base/fluxes/roeFlux.ipp:40-54, fastest at high
     Mach), so the batch is timed with the face-frame Roe over the configured 1000 steps (every case
     finite in FP64 and mixed) and the reference Roe over its finite window (50 steps) beside it."""
     import hyperboliconvergence_amd as hca
@@ -447,7 +447,7 @@ def main():
               f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})",
               file=sys.stderr, flush=True)
 
-    # the reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:84-91) and the cylinder case
+    # the reference's Roe dissipation mixes frames (base/fluxes/roeFlux.ipp:40-54) and the cylinder case
     # diverges after a few hundred steps in the reference algorithm itself: the mixed-vs-FP64 error is
     # measured over the window before the FP64 state leaves the finite numbers
     diverge = first_nonfinite_step(cfg, cfg["nsteps"]) if a.workload == "c2" and not a.no_extras else None

@@ -31,6 +31,8 @@ class TrajectorySolver:
         self.ctx.set_mesh(self.nodes)
         self.ctx.set_case(self.case)
         self.ctx.set_tiles(0)
+        # the context keeps the checkpoints of its latest forward pass only
+        self.generation = 0
 
     def set_minf(self, minf):
         minf = float(minf)
@@ -44,6 +46,7 @@ class TrajectorySolver:
         self.ctx.set_boundary_refs(bref)
         self.ctx.set_state(q0)
         self.ctx.adjoint_forward(self.nsteps, self.checkpoint_every)
+        self.generation += 1
         return self.ctx.get_state()
 
     def backward(self, qbar):
@@ -59,10 +62,14 @@ class _Trajectory(torch.autograd.Function):
         qK = solver.forward(q0.detach().cpu().double().numpy(), bref.detach().cpu().double().numpy(),
                             float(minf.detach().cpu()))
         ctx.solver = solver
+        ctx.generation = solver.generation
         return torch.from_numpy(qK).to(device=q0.device, dtype=q0.dtype)
 
     @staticmethod
     def backward(ctx, grad_qK):
+        if ctx.solver.generation != ctx.generation:
+            raise RuntimeError("trajectory: the solver ran another forward pass since this output was computed; "
+                               "its checkpoints belong to that pass (use one solver per pending backward)")
         q0b, bb, mb = ctx.solver.backward(grad_qK.detach().cpu().double().numpy())
         dev, dt = grad_qK.device, grad_qK.dtype
         return (torch.from_numpy(q0b).to(device=dev, dtype=dt), torch.from_numpy(bb).to(device=dev, dtype=dt),

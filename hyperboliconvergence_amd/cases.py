@@ -2,7 +2,7 @@
 descriptors of the benchmark configurations.
 
 Mesh generators restate make_linspace_mesh / make_cylinder_mesh
-(mesh/generate/twoD.h:178-203, 302-342) and add the geometrically stretched
+(mesh/generate/twoD.h:15-40, 139-179) and add the geometrically stretched
 O-grid of the cylinder cases.  Runge-Kutta tables are ODE::Explicit::ssp11/22/
 33/34 (ode.h:107-197) plus classical RK4 in the same (alpha, beta) form.
 """
@@ -96,7 +96,7 @@ def make_case(law, basis, flux, limiter, bc, rk, cfl, dt_mode=DT_GLOBAL, species
 # ---------------- meshes ----------------
 
 def mesh_linspace(ni, nj, lox, hix, loy, hiy):
-    """make_linspace_mesh (mesh/generate/twoD.h:178-203): nodes [ni+1][nj+1][2]."""
+    """make_linspace_mesh (mesh/generate/twoD.h:15-40): nodes [ni+1][nj+1][2]."""
     dx = (hix - lox) / ni
     dy = (hiy - loy) / nj
     i = np.arange(ni + 1, dtype=np.float64)[:, None]
@@ -131,7 +131,7 @@ def ogrid_ratio(nr, rc=0.5, rb=20.0, dr0=1e-3):
 def mesh_ogrid(nr, nth, rc=0.5, rb=20.0, dr0=1e-3):
     """O-grid around a cylinder, i = radius (geometric stretching, first spacing dr0),
     j = theta (uniform, node nth identical to node 0), as make_cylinder_mesh's layout
-    (mesh/generate/twoD.h:302-342)."""
+    (mesh/generate/twoD.h:139-179)."""
     s = ogrid_ratio(nr, rc, rb, dr0)
     i = np.arange(nr + 1)
     r = rc + dr0 * (np.power(s, i.astype(np.float64)) - 1.0) / (s - 1.0)
@@ -146,7 +146,7 @@ def mesh_ogrid(nr, nth, rc=0.5, rb=20.0, dr0=1e-3):
 
 
 def cell_centres(nodes):
-    """Vertex-average cell centres (geometry/volume.ipp:64-72)."""
+    """Vertex-average cell centres (geometry/volume.ipp:34-42)."""
     p0, p1, p2, p3 = nodes[:-1, :-1], nodes[1:, :-1], nodes[:-1, 1:], nodes[1:, 1:]
     return (((0.25 * p0 + 0.25 * p1) + 0.25 * p2) + 0.25 * p3)
 
@@ -159,7 +159,7 @@ def bref_size(ni, nj):
 
 def prim_to_cons(qp, gamma=GAMMA, R=287.058):
     """Conserved {rho u, rho v, rho, rho E} from primitive {u, v, rho, p}
-    (euler/transforms/primitive.ipp:235-271 then conserved.ipp:136-163)."""
+    (euler/transforms/primitive.ipp:24-60 then conserved.ipp:2-29)."""
     qp = np.asarray(qp, dtype=np.float64)
     u, v, r, p = qp[..., 0], qp[..., 1], qp[..., 2], qp[..., 3]
     k = 0.5 * (u * u + v * v)
@@ -288,7 +288,7 @@ def gresho_state(x, y, mach=0.3, vel_inf=0.1, density=1.0, gamma=GAMMA):
 
 def config_gresho(n=256, flux=FLUX_RUSANOV, limiter=LIM_NOLIMIT3, nsteps=10, cfl=1.6, mach=0.3, vel_inf=0.1,
                   density=1.0):
-    """The reference's Gresho driver (script/euler/gresho2D.cpp:68-110, 156-224): n x n cells over
+    """The reference's Gresho driver (script/euler/gresho2D.cpp:78-110, 161-224): n x n cells over
     [-1, 1]^2, periodic, Rusanov + NoLimit3 MUSCL, SSP(3,4), global dt at CFL 1.6, species minf = M.
     qref is the outer (rest) state used for the boundary fill and the output's reference values."""
     nodes = mesh_linspace(n, n, -1.0, 1.0, -1.0, 1.0)
@@ -304,7 +304,7 @@ def config_gresho(n=256, flux=FLUX_RUSANOV, limiter=LIM_NOLIMIT3, nsteps=10, cfl
 
 def config_soundwave(n, flux=FLUX_ROE, limiter=LIM_SMOOTHVANALBADA, mach=0.5, amplitude=0.01, nwavelengths=1,
                      cfl=0.5, crossings=1.0, rk="ssp34"):
-    """The forward-travelling sound wave of cases/euler/oneD/periodic.h:44-83 on an n x 4
+    """The forward-travelling sound wave of cases/euler/oneD/periodic.h:37-80 on an n x 4
     periodic strip over [0, 1]: background u0 = rho0 = 1, a0 = u0 / M, p0 = rho0 a0^2 / gamma;
     p = p0 (1 + M A sin(k x)), rho from the isentrope, u from the u - 2a/(gamma-1) invariant of
     the u + a wave.  Fixed step of cfl h / (u0 + a0) (the update divides by the reference's
@@ -361,7 +361,7 @@ def mesh_channelbump(d=2.0, l=3.0, w=2.0, h=0.1, nd=96, nl=48, nw=48):
 
 def config_channelbump(nd=96, nl=48, nw=48, flux=FLUX_RUSANOV, limiter=None, nsteps=6500, cfl=0.4, mach=0.4,
                        rk="ssp11"):
-    """The reference's channel-bump driver (script/euler/channelBump2D.cpp:58-140): d = 2, l = 3,
+    """The reference's channel-bump driver (script/euler/channelBump2D.cpp:80-140): d = 2, l = 3,
     w = 2, h = 0.1, free stream M = 0.4 (p = rho = 1) along x, fixed states at inflow and outflow
     (ids 0, 1), slip walls below and above (ids 2, 3), Rusanov + NoLimit1, forward Euler (ssp11),
     global dt at CFL 0.4, 6500 steps."""

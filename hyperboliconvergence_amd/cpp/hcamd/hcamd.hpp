@@ -214,7 +214,7 @@ template <floating_point Real> struct Mesh<2, Real> {
     std::array<Real, 2> period_j{0, 0};   // ... across boundary 3
 };
 
-// mesh/generate/twoD.h:178-203
+// mesh/generate/twoD.h:15-40
 template <floating_point Real>
 Mesh<2, Real> make_linspace_mesh(const std::array<std::size_t, 2>& cellDims, Real lox, Real hix, Real loy, Real hiy)
 {
@@ -230,7 +230,7 @@ Mesh<2, Real> make_linspace_mesh(const std::array<std::size_t, 2>& cellDims, Rea
     return m;
 }
 
-// mesh/generate/twoD.h:302-342 (uniform radial spacing, i = radius, j = theta)
+// mesh/generate/twoD.h:139-179 (uniform radial spacing, i = radius, j = theta)
 template <floating_point Real>
 Mesh<2, Real> make_cylinder_mesh(const std::array<Real, 2> lengths, const std::array<std::size_t, 2> ncells)
 {
@@ -433,7 +433,7 @@ template <LawType Law> struct RoeFlux : FluxInterface<RoeFlux<Law>, Law> {
     static constexpr int id = HC_FLUX_ROE, vscale = 0, pscale = 0;
 };
 // Roe flux with the Roe-average velocity rotated into the face frame (new; the reference's RoeFlux
-// keeps it in the global frame, base/fluxes/roeFlux.ipp:84-91)
+// keeps it in the global frame, base/fluxes/roeFlux.ipp:40-54)
 template <LawType Law> struct RoeFluxFaceFrame : FluxInterface<RoeFluxFaceFrame<Law>, Law> {
     static_assert(Law == LawType::Euler, "RoeFluxFaceFrame needs the characteristic basis of the Euler equations");
     static constexpr int id = HC_FLUX_ROE_FACEFRAME, vscale = 0, pscale = 0;

@@ -22,7 +22,7 @@
 // then j-faces nfi + i*(nj+1) + l.  On a periodic direction the last node line is the
 // first one and is skipped.  Semantics follow the stage kernel (hc_stage.hpp):
 // spatial/muscl.h:10-87, spatial/residualCalc.h:199-305, ghostCellLoop.h:133-224,
-// boundaryFluxLoop.h:150-261, spatial/lsqMetrics.h:269-301.
+// boundaryFluxLoop.h:150-261, spatial/lsqMetrics.h:181-213.
 #pragma once
 
 #include "hc_dual.hpp"

@@ -137,8 +137,11 @@ int hipfail(hc_context* c, hipError_t e, const char* where)
 }
 
 
+// Launches of the executable graph may still be in flight (hc_step returns without waiting), and
+// HIP does not promise to defer the destruction of an executing graph: wait for the stream first.
 void drop_graph(hc_context* ctx)
 {
+    if ((ctx->gexec || ctx->graph) && ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->gexec) (void)hipGraphExecDestroy(ctx->gexec);
     if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
     ctx->gexec = nullptr;

@@ -59,7 +59,7 @@ int build_geometry(const GeomInput& in, GeomOutput& out)
     const bool per_i = in.bc[0] == BC_PERIODIC;
     const bool per_j = in.bc[2] == BC_PERIODIC;
 
-    // cells: volume.ipp:55-96 evaluated at the element centre (shape values 1/4,
+    // cells: volume.ipp:25-66 evaluated at the element centre (shape values 1/4,
     // derivative rows (-1/4,-1/4), (1/4,-1/4), (-1/4,1/4), (1/4,1/4))
     int bad = 0;
     for (int i = 0; i < ni; ++i)
@@ -98,7 +98,7 @@ int build_geometry(const GeomInput& in, GeomOutput& out)
         x[2] = x[2] + dy * dy;
     };
     // spatial metric, per cell: interior i-edges, interior j-edges, then the boundary
-    // sides (periodic image edge, or the mirror point of lsqMetrics.h:204-212)
+    // sides (periodic image edge, or the mirror point of lsqMetrics.h:116-124)
     for (int j = 0; j < nj; ++j)
         for (int i = 0; i < ni - 1; ++i) {
             const double dx = cx(i + 1, j) - cx(i, j), dy = cy(i + 1, j) - cy(i, j);
@@ -188,7 +188,7 @@ int build_geometry(const GeomInput& in, GeomOutput& out)
                 muscl_face(fi, nfi, idx, s, cx(l - 1, j), cy(l - 1, j), l - 1, j, l, j);
             } else if (per_i) {
                 if (l == 0) {
-                    // periodicLoop.h:277-300: wrap face on node line 0, left cell is the image of ni-1
+                    // periodicLoop.h:129-152: wrap face on node line 0, left cell is the image of ni-1
                     const double lx = cx(ni - 1, j) - in.period_i[0], ly = cy(ni - 1, j) - in.period_i[1];
                     muscl_face(fi, nfi, idx, s, lx, ly, ni - 1, j, 0, j);
                 }

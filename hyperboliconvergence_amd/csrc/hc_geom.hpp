@@ -1,9 +1,9 @@
 // hc_geom.hpp -- one-time geometry of a structured mesh for the stage kernel:
-// cell volumes and centres (geometry/volume.ipp:55-96, mesh/dual.ipp:121-162),
+// cell volumes and centres (geometry/volume.ipp:25-66, mesh/dual.ipp:42-83),
 // face surfaces (geometry/surface.ipp:12-29), the LSQ spatial metric
-// (spatial/lsqMetrics.h:119-251) and, per face, everything of the MUSCL
+// (spatial/lsqMetrics.h:31-163) and, per face, everything of the MUSCL
 // reconstruction that depends on geometry only (spatial/muscl.h:35-50,
-// lsq/factor_solve.ipp:403-490): distances d_L, d_R, 1/(d_L+d_R), the centre
+// lsq/factor_solve.ipp:13-100): distances d_L, d_R, 1/(d_L+d_R), the centre
 // difference dx_c and the factored left/right biased metrics.
 #pragma once
 

@@ -70,7 +70,7 @@ template <class R> struct EState {
     R p, r, t, h, v2, a2, u, v;
 };
 
-// State<ScalarAdvection,2>, conservationLaws/scalarAdvection/scalarAdvection.h:136-147
+// State<ScalarAdvection,2>, conservationLaws/scalarAdvection/scalarAdvection.h:48-59
 template <class R> struct SState {
     R q, u, v;
 };
@@ -101,7 +101,7 @@ template <class R> HC_HD R std_clamp(R v, R lo, R hi) { return (v < lo) ? lo : (
 
 // ---------------- set <-> state ----------------
 
-// euler/transforms/conserved.ipp:165-210
+// euler/transforms/conserved.ipp:31-76
 template <class R> HC_HD EState<R> euler_from_conserved(const Species<R>& sp, const R* qc)
 {
     EState<R> s;
@@ -130,7 +130,7 @@ template <class R> HC_HD EState<R> euler_from_conserved(const Species<R>& sp, co
     return s;
 }
 
-// euler/transforms/primitive.ipp:235-271
+// euler/transforms/primitive.ipp:24-60
 template <class R> HC_HD EState<R> euler_from_primitive(const Species<R>& sp, const R* qp)
 {
     EState<R> s;
@@ -157,7 +157,7 @@ template <class R> HC_HD EState<R> euler_from_primitive(const Species<R>& sp, co
     return s;
 }
 
-// euler/transforms/conserved.ipp:136-163
+// euler/transforms/conserved.ipp:2-29
 template <class R> HC_HD void euler_to_conserved(const EState<R>& s, R* qc)
 {
     const R re = s.r * s.h - s.p;
@@ -167,7 +167,7 @@ template <class R> HC_HD void euler_to_conserved(const EState<R>& s, R* qc)
     qc[3] = re;
 }
 
-// euler/transforms/primitive.ipp:216-232
+// euler/transforms/primitive.ipp:5-21
 template <class R> HC_HD void euler_to_primitive(const EState<R>& s, R* qp)
 {
     qp[0] = s.u;
@@ -182,7 +182,7 @@ template <class R, int BASIS> HC_HD EState<R> euler_state(const Species<R>& sp, 
     else return euler_from_conserved(sp, q);
 }
 
-// scalarAdvection/transforms/conserved.ipp:244-259
+// scalarAdvection/transforms/conserved.ipp:19-34
 template <class R> HC_HD SState<R> scalar_state(const R* q)
 {
     SState<R> s;
@@ -215,7 +215,7 @@ template <class R, int LAW, int BASIS> HC_HD void from_conserved(const Species<R
 
 // ---------------- delta transforms about a state ----------------
 
-// euler/transforms/primitive.ipp:276-318
+// euler/transforms/primitive.ipp:65-107
 template <class R> HC_HD void dcons_to_dprim(const Species<R>& sp, const EState<R>& s, const R* dqc, R* dqp)
 {
     const R dr = dqc[2];
@@ -240,7 +240,7 @@ template <class R> HC_HD void dcons_to_dprim(const Species<R>& sp, const EState<
     dqp[3] = dp;
 }
 
-// euler/transforms/primitive.ipp:321-367
+// euler/transforms/primitive.ipp:110-156
 template <class R> HC_HD void dprim_to_dcons(const Species<R>& sp, const EState<R>& s, const R* dqp, R* dqc)
 {
     const R dr = dqp[2];
@@ -333,7 +333,7 @@ template <class R> HC_HD R proj_vel(R n0, R n1, R u, R v)
 
 // ---------------- exact fluxes ----------------
 
-// euler/fluxes/exactFlux.ipp:22-54
+// euler/fluxes/exactFlux.ipp:4-36
 template <class R> HC_HD Flux4<R> euler_exact(const FaceG<R>& f, const EState<R>& s)
 {
     Flux4<R> fr;
@@ -348,7 +348,7 @@ template <class R> HC_HD Flux4<R> euler_exact(const FaceG<R>& f, const EState<R>
     return fr;
 }
 
-// scalarAdvection/fluxes/exactFlux.ipp:211-224
+// scalarAdvection/fluxes/exactFlux.ipp:8-21
 template <class R> HC_HD Flux4<R> scalar_exact(const FaceG<R>& f, const SState<R>& s)
 {
     Flux4<R> fr;
@@ -369,7 +369,7 @@ template <class R> HC_HD R euler_specrad(R n0, R n1, const EState<R>& s)
 
 // ---------------- flux functors ----------------
 
-// base/fluxes/centralFlux.ipp:2-14 (CentralFlux<Law>::flux, roeFlux.ipp:25-37)
+// base/fluxes/centralFlux.ipp:2-14 (CentralFlux<Law>::flux, roeFlux.ipp:11-12)
 template <class R> HC_HD Flux4<R> euler_central(const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
 {
     const Flux4<R> fl = euler_exact(f, sl);
@@ -391,7 +391,7 @@ template <class R> HC_HD Flux4<R> scalar_central(const FaceG<R>& f, const SState
     return o;
 }
 
-// base/fluxes/rusanovFlux.ipp:99-147
+// base/fluxes/rusanovFlux.ipp:2-50
 template <class R> HC_HD Flux4<R> euler_rusanov(const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
 {
     Flux4<R> o = euler_central(f, sl, sr);
@@ -417,7 +417,7 @@ template <class R> HC_HD Flux4<R> scalar_rusanov(const FaceG<R>& f, const SState
     return o;
 }
 
-// euler/fluxes/roe.ipp:57-103
+// euler/fluxes/roe.ipp:2-48
 template <class R> HC_HD EState<R> roe_average(const Species<R>& sp, const EState<R>& sl, const EState<R>& sr)
 {
     EState<R> a;
@@ -457,7 +457,7 @@ template <class R> HC_HD void wavespeeds(const FaceG<R>& f, const EState<R>& s, 
     w[3] = u;
 }
 
-// euler/fluxes/roe.ipp:105-133 (Harten entropy fix, eps = 0.05)
+// euler/fluxes/roe.ipp:50-78 (Harten entropy fix, eps = 0.05)
 template <class R> HC_HD void entropyfix(const R* la, const R* ll, const R* lr, R* s)
 {
     const R ua = hc_fabs(la[0]);
@@ -476,10 +476,10 @@ template <class R> HC_HD void entropyfix(const R* la, const R* ll, const R* lr, 
     }
 }
 
-// base/fluxes/roeFlux.ipp:39-95 (RoeDissipation): fd = R |Lambda| L (dq) in the global frame for the
+// base/fluxes/roeFlux.ipp:2-58 (RoeDissipation): fd = R |Lambda| L (dq) in the global frame for the
 // conserved jump dq, returns the Roe-state spectral radius times the area.  FACEFRAME = true is the
 // added variant that rotates the Roe-average velocity into the face frame like the jump; the
-// reference keeps it in the global frame (roeFlux.ipp:84-91).
+// reference keeps it in the global frame (roeFlux.ipp:40-54).
 template <class R, bool FACEFRAME = false>
 HC_HD R roe_dissipation(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr, const R* dq,
                         R* fd)
@@ -515,7 +515,7 @@ HC_HD R roe_dissipation(const Species<R>& sp, const FaceG<R>& f, const EState<R>
     return euler_specrad(f.n0, f.n1, ravg) * f.area;
 }
 
-// base/fluxes/roeFlux.ipp:39-95 (RoeFlux = CentralFlux + RoeDissipation)
+// base/fluxes/roeFlux.ipp:2-58 (RoeFlux = CentralFlux + RoeDissipation)
 template <class R, bool FACEFRAME = false>
 HC_HD Flux4<R> euler_roe(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
 {
@@ -629,7 +629,7 @@ HC_HD Flux4<double> euler_roe_unprec_ws_mixed(const Species<float>& spf, const F
     return o;
 }
 
-// euler/fluxes/machSplittings.ipp:179-214
+// euler/fluxes/machSplittings.ipp:8-43
 template <class R> HC_HD R ms_m1(int sign, R M) { return R(0.5) * (M + R(sign) * hc_fabs(M)); }
 template <class R> HC_HD R ms_m2(int sign, R M) { return R(sign) * R(0.25) * (M + R(sign)) * (M + R(sign)); }
 template <class R> HC_HD R ms_m4(int sign, R M, R beta)
@@ -647,7 +647,7 @@ template <class R> HC_HD R ms_p5(int sign, R M, R alpha)
     return ms_p3(sign, M) - R(sign) * R(16.) * alpha * M * ms_m2(sign, M) * ms_m2(-sign, M);
 }
 
-// euler/fluxes/slau.ipp:131-288 (SLAU mass flux, SLAU2 pressure flux)
+// euler/fluxes/slau.ipp:4-161 (SLAU mass flux, SLAU2 pressure flux)
 template <class R> HC_HD Flux4<R> euler_slau(const Species<R>& sp, const FaceG<R>& f, const EState<R>& sl, const EState<R>& sr)
 {
     const R ua = hc_sqrt(R(0.5) * (sl.v2 + sr.v2));
@@ -861,7 +861,7 @@ template <class R> HC_HD R limit(int lim, R a, R b)
 
 // ---------------- boundary point functions ----------------
 
-// Inviscid wall flux with q_wall = q_interior, euler/boundary/inviscidWall.ipp:230-269.
+// Inviscid wall flux with q_wall = q_interior, euler/boundary/inviscidWall.ipp:33-72.
 // f is the surface with its normal into the domain.
 template <class R, int BASIS> HC_HD Flux4<R> euler_wall_flux(const Species<R>& sp, const FaceG<R>& f, const R* qint)
 {
@@ -893,7 +893,7 @@ template <class R, int BASIS> HC_HD void prim_to_solution(const Species<R>& sp, 
     }
 }
 
-// euler/boundary/riemann.ipp:102-197 (f: normal into the domain).  Returns 0 when
+// euler/boundary/riemann.ipp:18-113 (f: normal into the domain).  Returns 0 when
 // the face normal does not point into the domain (the reference asserts).
 template <class R, int BASIS> HC_HD int euler_riemann(const Species<R>& sp, const FaceG<R>& f, R fcx, R fcy, R ccx, R ccy,
                                                       const R* qint, const R* qfar, R* out)
@@ -991,7 +991,7 @@ template <class R, int BASIS> HC_HD int euler_entropy(const Species<R>& sp, cons
     return ok;
 }
 
-// scalarAdvection/boundary/riemann.ipp:275-300
+// scalarAdvection/boundary/riemann.ipp:15-40
 template <class R> HC_HD void scalar_riemann(const FaceG<R>& f, const R* qint, const R* qref, R* out)
 {
     R avg[3], rot[3];
@@ -1026,7 +1026,7 @@ HC_HD int ghost_value(const Species<R>& sp, int bctype, const FaceG<R>& f, R fcx
 
 // ---------------- LSQ (lsq/factor_solve.ipp) ----------------
 
-// lsq::factor for nDim = 2 (factor_solve.ipp:403-412): inverse of {x00, x01, x11}
+// lsq::factor for nDim = 2 (factor_solve.ipp:13-22): inverse of {x00, x01, x11}
 template <class R> HC_HD void lsq_factor(const R* xm, R* xf)
 {
     const R d = R(1.) / (xm[0] * xm[2] - xm[1] * xm[1]);
@@ -1035,7 +1035,7 @@ template <class R> HC_HD void lsq_factor(const R* xm, R* xf)
     xf[2] = d * xm[0];
 }
 
-// lsq::solve for nDim = 2 (factor_solve.ipp:443-459): gradient along dir
+// lsq::solve for nDim = 2 (factor_solve.ipp:47-69): gradient along dir
 template <class R> HC_HD R lsq_solve1(const R* xf, R q0, R q1, R d0, R d1)
 {
     return (xf[0] * q0 + xf[1] * q1) * d0 + (xf[1] * q0 + xf[2] * q1) * d1;

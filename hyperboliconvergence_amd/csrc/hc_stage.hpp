@@ -6,7 +6,7 @@
 //   1. load   : solution variables of the tile plus a 2-cell halo into LDS
 //               (periodic wrap across periodic boundaries)
 //   2. qmetric: LSQ solution metric Q = sum dq (x) dx for the tile plus a
-//               1-cell ring (spatial/lsqMetrics.h:269-301)
+//               1-cell ring (spatial/lsqMetrics.h:181-213)
 //   3. faces  : every i- and j-face touching the tile: MUSCL reconstruction
 //               with LSQ-biased gradients and the limiter, then the flux
 //               functor (spatial/muscl.h:10-87); boundary faces apply the ghost
@@ -350,7 +350,7 @@ struct StageTile {
     }
 
     // gradient along n from the biased LSQ metric: the factor solve on FP64 slots
-    // (lsq/factor_solve.ipp:443-459), the directional weights on the FP32 slots
+    // (lsq/factor_solve.ipp:47-69), the directional weights on the FP32 slots
     HC_HD static RF directional(const RF* f, RF q0, RF q1, RF n0, RF n1)
     {
         if constexpr (sizeof(RF) == sizeof(float)) return q0 * f[0] + q1 * f[1];
@@ -642,7 +642,7 @@ struct StageTile {
         const double vol = CPF ? ci.vol : a.m.cvol[c];
         if (a.mode == MODE_GLOBAL_STAGE0) {
             for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
-            red = lam / vol;   // spectralRadius.h:399-401
+            red = lam / vol;   // spectralRadius.h:33-35
             return;
         }
         double fac;
@@ -666,7 +666,7 @@ struct StageTile {
         update_cell(a, c, q0, Rt, fac, flag);
     }
 
-    // q_out = S( C(q0) + R_tot*fac )   (spatial/eulerForwardUpdate.h:274-283)
+    // q_out = S( C(q0) + R_tot*fac )   (spatial/eulerForwardUpdate.h:76-85)
     HC_HD static void update_cell(const StageArgs<RF>& a, long long c, const double* q0, const double* Rt, double fac,
                                   unsigned int& flag)
     {

@@ -3,7 +3,7 @@ drivers, so that their gnuplot scripts (cpp/cyl.gnp and the like) read this pack
 
   write_cylinder_result  script/euler/cylinder2D.cpp:33-63, 192-236   (theta z r + 6 state columns)
   write_mesh             script/euler/cylinder2D.cpp:238-272          (node x y, a block per row)
-  write_gresho_result    script/euler/gresho2D.cpp:22-50, 226-262     (x y + 7 state columns)
+  write_gresho_result    script/euler/gresho2D.cpp:34-74, 226-265     (x y + 7 state columns)
   write_shocktube_result script/euler/shocktube.cpp:60-77, 108-133    (5 columns per cell)
   write_scalar_result    script/scalar/cartesian2D.cpp:76-93, 147-170 (x y  u v q)
 

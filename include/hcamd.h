@@ -92,7 +92,7 @@ const char* hc_last_error(const hc_context* ctx);
 
 /*
  * Mesh nodes of a structured 2-D mesh; replaces Mesh<2,Real> construction and
- * dual() (mesh/mesh.h:19-57, mesh/dual.ipp:121-162).  Geometry is derived on
+ * dual() (mesh/mesh.h:19-57, mesh/dual.ipp:42-83).  Geometry is derived on
  * the next hc_set_case.
  */
 int hc_set_mesh(hc_context* ctx, int ni, int nj, const double* nodes);
@@ -101,7 +101,7 @@ int hc_set_mesh(hc_context* ctx, int ni, int nj, const double* nodes);
  * Flux functor, limiter, boundary conditions, Runge-Kutta table, CFL and
  * species; builds the one-time geometry (cells, faces, LSQ metrics) and
  * replaces make_muscl_flux (spatial/muscl.h:10-87), xmetrics
- * (spatial/lsqMetrics.h:108-150) and the BC tuple.
+ * (spatial/lsqMetrics.h:20-62) and the BC tuple.
  */
 int hc_set_case(hc_context* ctx, const hc_case_desc* desc);
 
@@ -118,14 +118,14 @@ int hc_get_state(hc_context* ctx, double* q);
 /*
  * Spatial residual of the current state, FluxResult per cell {flux[nvar], lambda}:
  * boundaryUpdate + qmetrics + residualCalc (spatial/boundary/boundaryUpdate.h:28-48,
- * spatial/lsqMetrics.h:269-301, spatial/residualCalc.h:45-69).
+ * spatial/lsqMetrics.h:181-213, spatial/residualCalc.h:45-69).
  */
 int hc_residual(hc_context* ctx, double* res);
 
 /*
  * Advance nsteps time steps: the body of integrate() (timestepping/rungeKutta.h:44-167)
  * with rungeKuttaAccumulation (:176-197), spectralRadius (spatial/spectralRadius.h:24-48)
- * and eulerForwardUpdateGlobal / Local (spatial/eulerForwardUpdate.h:236-366).
+ * and eulerForwardUpdateGlobal / Local (spatial/eulerForwardUpdate.h:38-168).
  * Asynchronous with respect to the host (hc_synchronize or any getter waits).
  */
 int hc_step(hc_context* ctx, int nsteps);

@@ -21,20 +21,20 @@
  *    face.metric[0], ql/qr read as q_l/q_r).
  *  - LSQ neighbours are the four face neighbours; across a periodic boundary
  *    the neighbour is the periodic image (centre translated by the period
- *    vector); mirror points (spatial/lsqMetrics.h:194-251) are added to the
+ *    vector); mirror points (spatial/lsqMetrics.h:104-163) are added to the
  *    X metric only on non-periodic sides; the Q metric has no boundary term on
- *    non-periodic sides (spatial/lsqMetrics.h:269-301 accumulates interior
+ *    non-periodic sides (spatial/lsqMetrics.h:181-213 accumulates interior
  *    edges only).  Per cell the order is: interior i-edges, interior j-edges,
  *    boundary i side, boundary j side.
  *  - Periodic wrap faces are full MUSCL faces on the i=0 / j=0 node faces
- *    (spatial/boundary/periodicLoop.h:277-324) with the left cell replaced by
+ *    (spatial/boundary/periodicLoop.h:129-176) with the left cell replaced by
  *    its periodic image.
  *  - Ghost-cell faces are first order F(q_ghost, q_int) / F(q_int, q_ghost)
  *    (spatial/boundary/ghostCellLoop.h:133-224).
  *  - Inviscid wall flux uses q_wall = q_interior (dq = 0 in
- *    conservationLaws/euler/boundary/inviscidWall.ipp:51-52).
+ *    conservationLaws/euler/boundary/inviscidWall.ipp:54-57).
  *  - Local time stepping: dq = R_tot * ((beta_s*cfl)/lambda0_i), lambda0 being
- *    the stage-0 spectral radius of the cell (spatial/eulerForwardUpdate.h:353-358
+ *    the stage-0 spectral radius of the cell (spatial/eulerForwardUpdate.h:155-160
  *    called with cfl -> beta_s*cfl, and the lambda of the stage-0 residual).
  */
 #include "oracle.h"
@@ -110,7 +110,7 @@ Face surface(const double* p0, const double* p1)
     return f;
 }
 
-// geometry/volume.ipp:55-96 with fem/fem.h:38-77 evaluated at the element centre
+// geometry/volume.ipp:25-66 with fem/fem.h:38-77 evaluated at the element centre
 Cell volume(const double* p0, const double* p1, const double* p2, const double* p3)
 {
     // fem::n2({0,0}) = {0.25,0.25,0.25,0.25}; fem::dn2({0,0}) rows:
@@ -153,7 +153,7 @@ inline Species to_species(const orc_species& s)
     return Species{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
 }
 
-// euler/transforms/conserved.ipp:165-210
+// euler/transforms/conserved.ipp:31-76
 void euler_cons2state(const Species& sp, const double* qc, double* st)
 {
     const double gam = sp.gamma;
@@ -180,7 +180,7 @@ void euler_cons2state(const Species& sp, const double* qc, double* st)
     st[SA2] = a2;
 }
 
-// euler/transforms/primitive.ipp:235-271
+// euler/transforms/primitive.ipp:24-60
 void euler_prim2state(const Species& sp, const double* qp, double* st)
 {
     const double gam = sp.gamma;
@@ -205,7 +205,7 @@ void euler_prim2state(const Species& sp, const double* qp, double* st)
     st[SA2] = a2;
 }
 
-// euler/transforms/conserved.ipp:136-163
+// euler/transforms/conserved.ipp:2-29
 void euler_state2cons(const double* st, double* qc)
 {
     const double r = st[SR];
@@ -218,7 +218,7 @@ void euler_state2cons(const double* st, double* qc)
     qc[3] = re;
 }
 
-// euler/transforms/primitive.ipp:216-232
+// euler/transforms/primitive.ipp:5-21
 void euler_state2prim(const double* st, double* qp)
 {
     qp[0] = st[SU];
@@ -227,7 +227,7 @@ void euler_state2prim(const double* st, double* qp)
     qp[3] = st[SP];
 }
 
-// scalarAdvection/transforms/conserved.ipp:244-259 (set2State), :227-242 (state2Set)
+// scalarAdvection/transforms/conserved.ipp:19-34 (set2State), :2-17 (state2Set)
 void scalar_cons2state(const double* qc, double* st)
 {
     st[SQU] = qc[0];
@@ -289,7 +289,7 @@ void cons2sol(const orc_case& cs, const Species& sp, const double* qc, double* q
     euler_state2prim(st, q);
 }
 
-// euler/transforms/primitive.ipp:276-318 conserved delta -> primitive delta
+// euler/transforms/primitive.ipp:65-107 conserved delta -> primitive delta
 void d_cons2prim(const Species& sp, const double* st, const double* dqc, double* dqp)
 {
     const double dr = dqc[2];
@@ -309,7 +309,7 @@ void d_cons2prim(const Species& sp, const double* st, const double* dqc, double*
     dqp[3] = dp;
 }
 
-// euler/transforms/primitive.ipp:321-367 primitive delta -> conserved delta
+// euler/transforms/primitive.ipp:110-156 primitive delta -> conserved delta
 void d_prim2cons(const Species& sp, const double* st, const double* dqp, double* dqc)
 {
     const double dr = dqp[2];
@@ -397,7 +397,7 @@ inline double projected_velocity(const double* n, const double* st, int uoff)
 
 /* ---------------- fluxes ---------------- */
 
-// euler/fluxes/exactFlux.ipp:22-54 (Euler) and scalarAdvection/fluxes/exactFlux.ipp:211-224
+// euler/fluxes/exactFlux.ipp:4-36 (Euler) and scalarAdvection/fluxes/exactFlux.ipp:8-21
 FluxRes exact_flux(const orc_case& cs, const double* n, const double* st)
 {
     FluxRes fr{};
@@ -423,7 +423,7 @@ FluxRes exact_flux(const orc_case& cs, const double* n, const double* st)
     return fr;
 }
 
-// euler/euler.h:161-166, scalarAdvection/scalarAdvection.h:181-186
+// euler/euler.h:161-166, scalarAdvection/scalarAdvection.h:93-98
 inline double spectral_radius(const orc_case& cs, const double* n, const double* st)
 {
     if (cs.law == ORC_LAW_SCALAR) return std::abs(projected_velocity(n, st, SQU));
@@ -433,7 +433,7 @@ inline double spectral_radius(const orc_case& cs, const double* n, const double*
 // std::max as used by the reference (returns the first argument unless it is less)
 inline double stdmax(double a, double b) { return (a < b) ? b : a; }
 
-// base/fluxes/centralFlux.ipp:2-14 (as CentralFlux<Law>::flux, roeFlux.ipp:25-37)
+// base/fluxes/centralFlux.ipp:2-14 (as CentralFlux<Law>::flux, roeFlux.ipp:11-12)
 FluxRes central_flux(const orc_case& cs, const Face& f, const double* sl, const double* sr)
 {
     const int nv = nvar_of(cs.law);
@@ -451,7 +451,7 @@ void state2cons_law(const orc_case& cs, const double* st, double* qc)
     else euler_state2cons(st, qc);
 }
 
-// base/fluxes/rusanovFlux.ipp:120-147
+// base/fluxes/rusanovFlux.ipp:23-50
 FluxRes rusanov_dissipation(const orc_case& cs, const Face& f, const double* sl, const double* sr)
 {
     const int nv = nvar_of(cs.law);
@@ -466,7 +466,7 @@ FluxRes rusanov_dissipation(const orc_case& cs, const Face& f, const double* sl,
     return out;
 }
 
-// base/fluxes/rusanovFlux.ipp:99-117
+// base/fluxes/rusanovFlux.ipp:2-20
 FluxRes rusanov_flux(const orc_case& cs, const Face& f, const double* sl, const double* sr)
 {
     const int nv = nvar_of(cs.law);
@@ -476,7 +476,7 @@ FluxRes rusanov_flux(const orc_case& cs, const Face& f, const double* sl, const 
     return result;
 }
 
-// euler/fluxes/roe.ipp:57-103
+// euler/fluxes/roe.ipp:2-48
 void roe_average(const Species& sp, const double* sl, const double* sr, double* ravg)
 {
     const double cr = std::sqrt(sr[SR] / sl[SR]);
@@ -513,7 +513,7 @@ void wavespeeds(const double* n, const double* st, double* w)
     w[3] = u;
 }
 
-// euler/fluxes/roe.ipp:105-133
+// euler/fluxes/roe.ipp:50-78
 void entropyfix(const double* la, const double* ll, const double* lr, double* s)
 {
     const double ua = std::fabs(la[0]);
@@ -529,7 +529,7 @@ void entropyfix(const double* la, const double* ll, const double* lr, double* s)
     }
 }
 
-// base/fluxes/roeFlux.ipp:59-95.  face_frame = true is the added variant whose Roe-average
+// base/fluxes/roeFlux.ipp:22-58.  face_frame = true is the added variant whose Roe-average
 // velocity is rotated into the face frame like the jump (the reference keeps it in the global frame).
 FluxRes roe_dissipation(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr,
                         bool face_frame = false)
@@ -622,7 +622,7 @@ FluxRes roe_unprec_ws_flux(const orc_case& cs, const Species& sp, const Face& f,
     return out;
 }
 
-// euler/fluxes/machSplittings.ipp:179-214
+// euler/fluxes/machSplittings.ipp:8-43
 inline double msM1(int sign, double M) { return 0.5 * (M + sign * std::fabs(M)); }
 inline double msM2(int sign, double M) { return sign * 0.25 * (M + sign) * (M + sign); }
 inline double msM4(int sign, double M, double beta) { return msM2(sign, M) * (1. - sign * 16. * beta * msM2(-sign, M)); }
@@ -639,7 +639,7 @@ inline double msP5(int sign, double M, double alpha)
 
 inline double stdclamp(double v, double lo, double hi) { return (v < lo) ? lo : (hi < v) ? hi : v; }
 
-// euler/fluxes/slau.ipp:131-288
+// euler/fluxes/slau.ipp:4-161
 FluxRes slau_flux(const orc_case& cs, const Species& sp, const Face& f, const double* sl, const double* sr)
 {
     const double ua = std::sqrt(0.5 * (sl[SV2] + sr[SV2]));
@@ -653,7 +653,7 @@ FluxRes slau_flux(const orc_case& cs, const Species& sp, const Face& f, const do
     const double mr = unr * aa1;
     const double una = (sl[SR] * std::fabs(unl) + sr[SR] * std::fabs(unr)) / (sl[SR] + sr[SR]);
     const double chi = (1. - ma_hat) * (1. - ma_hat);
-    // m0_c, m0_a and the scaling choices do not enter the SLAU/SLAU2 terms used (slau.ipp:164-191)
+    // m0_c, m0_a and the scaling choices do not enter the SLAU/SLAU2 terms used (slau.ipp:37-64)
     const double g = -std::fmax(std::fmin(ml, 0.), -1.) * std::fmin(std::fmax(mr, 0.), 1.);
     const double dr = sr[SR] - sl[SR];
     const double dp = sr[SP] - sl[SP];
@@ -772,23 +772,23 @@ double limit(int lim, double a, double b)
         return 0.5 * (a + b);
     case ORC_LIM_NOLIMIT3:   // limiter.ipp:155-159
         return 0.33333333333333333 * (2. * a + b);
-    case ORC_LIM_VANALBADA2: {   // limiter.ipp:161-172
+    case ORC_LIM_VANALBADA2: {   // limiter.ipp:26-37
         const double eps = std::numeric_limits<double>::min();
         return a * b < 0 ? 0 : (b * (a * a) + a * (b * b)) / (a * a + b * b + eps);
     }
-    case ORC_LIM_MINMOD2:   // limiter.ipp:174-182
+    case ORC_LIM_MINMOD2:   // limiter.ipp:39-47
         return a > 0 ? std::fmin(a, std::fmax(b, 0.)) : std::fmax(a, std::fmin(b, 0.));
-    case ORC_LIM_SUPERBEE2:   // limiter.ipp:184-197
+    case ORC_LIM_SUPERBEE2:   // limiter.ipp:49-62
         return a > 0 ? std::fmax(0., std::fmax(std::fmin(2 * a, b), std::fmin(2 * b, a)))
                      : std::fmin(0., std::fmin(std::fmax(2 * a, b), std::fmax(2 * b, a)));
-    case ORC_LIM_MC2:   // limiter.ipp:199-214
+    case ORC_LIM_MC2:   // limiter.ipp:64-79
         return a > 0 ? std::fmax(0., std::fmin(0.5 * (a + b), std::fmin(2 * b, 2 * a)))
                      : std::fmin(0., std::fmax(0.5 * (a + b), std::fmax(2 * b, 2 * a)));
-    case ORC_LIM_VANLEER2: {   // limiter.ipp:216-224
+    case ORC_LIM_VANLEER2: {   // limiter.ipp:81-89
         const double eps = std::numeric_limits<double>::min();
         return (b + std::fabs(b)) / (a + std::fabs(b) + eps);
     }
-    case ORC_LIM_CADA3: {   // limiter.ipp:226-296
+    case ORC_LIM_CADA3: {   // limiter.ipp:91-161
         double t = 0.0;
         double gam = 1.5;
         double c1 = -2. / 7.;
@@ -812,7 +812,7 @@ double limit(int lim, double a, double b)
         if (a <= 0) w = -w;
         return w;
     }
-    default: {   // smooth van Albada: limiter.ipp:161-172 without the a*b<0 switch
+    default: {   // smooth van Albada: limiter.ipp:26-37 without the a*b<0 switch
         const double eps = std::numeric_limits<double>::min();
         return (b * (a * a) + a * (b * b)) / (a * a + b * b + eps);
     }
@@ -821,8 +821,8 @@ double limit(int lim, double a, double b)
 
 /* ---------------- boundary point functions ---------------- */
 
-// euler/fluxes/roe.ipp:57-103 applied to VariableSets (base/base.h:346-360)
-// euler/boundary/riemann.ipp:102-197
+// euler/fluxes/roe.ipp:2-48 applied to VariableSets (base/base.h:346-360)
+// euler/boundary/riemann.ipp:18-113
 int euler_riemann_update(const orc_case& cs, const Species& sp, const Face& f, const Cell& c,
                          const double* qint, const double* qfar, double* out)
 {
@@ -923,7 +923,7 @@ int euler_entropy_update(const orc_case& cs, const Species& sp, const Face& f, c
     return ok;
 }
 
-// scalarAdvection/boundary/riemann.ipp:275-300
+// scalarAdvection/boundary/riemann.ipp:15-40
 int scalar_riemann_update(const orc_case& cs, const Face& f, const double* qint, const double* qref, double* out)
 {
     double avg[3], rot[3], st[8];
@@ -935,7 +935,7 @@ int scalar_riemann_update(const orc_case& cs, const Face& f, const double* qint,
     return 1;
 }
 
-// spatial/boundary/boundaryCondition.h:63-80 (Fixed), euler/boundary/inviscidWall.ipp:212-225 (wall update)
+// spatial/boundary/boundaryCondition.h:63-80 (Fixed), euler/boundary/inviscidWall.ipp:15-28 (wall update)
 int bc_update(const orc_case& cs, const Species& sp, int type, const Face& f, const Cell& c,
               const double* qint, const double* qref, double* out)
 {
@@ -949,7 +949,7 @@ int bc_update(const orc_case& cs, const Species& sp, int type, const Face& f, co
     return 1;
 }
 
-// euler/boundary/inviscidWall.ipp:230-269 with dq = 0
+// euler/boundary/inviscidWall.ipp:33-72 with dq = 0
 FluxRes wall_flux(const orc_case& cs, const Species& sp, const Face& f, const double* qint)
 {
     double st[8];
@@ -967,7 +967,7 @@ FluxRes wall_flux(const orc_case& cs, const Species& sp, const Face& f, const do
 
 struct XM { double x[3]; };
 
-// lsq/xmetric.ipp:302-318
+// lsq/xmetric.ipp:53-69
 inline XM xmetric(const double* dx)
 {
     XM xm;
@@ -977,7 +977,7 @@ inline XM xmetric(const double* dx)
     return xm;
 }
 
-// lsq/factor_solve.ipp:403-412 then :443-459
+// lsq/factor_solve.ipp:13-22 then :47-69
 void lsq_solve(const double* xm, const double* qm, int nv, const double* dir, double* dq)
 {
     const double d = 1. / (xm[0] * xm[2] - xm[1] * xm[1]);
@@ -1004,7 +1004,7 @@ struct Mesh {
 void build_cells(Mesh& m)
 {
     m.cells.resize((size_t)m.ni * m.nj);
-    // mesh/dual.ipp:121-162, corners o0{0,0} o1{1,0} o2{0,1} o3{1,1}
+    // mesh/dual.ipp:42-83, corners o0{0,0} o1{1,0} o2{0,1} o3{1,1}
     for (int i = 0; i < m.ni; ++i)
         for (int j = 0; j < m.nj; ++j)
             m.cells[(size_t)i * m.nj + j] = volume(m.node(i, j), m.node(i + 1, j), m.node(i, j + 1), m.node(i + 1, j + 1));
@@ -1018,7 +1018,7 @@ inline Cell image_of(const Cell& c, const double* period)
     return im;
 }
 
-// spatial/lsqMetrics.h:119-150, 192-251 and parallalg/neighbour_algorithm.h:253-317
+// spatial/lsqMetrics.h:31-62, 104-163 and parallalg/neighbour_algorithm.h:253-317
 void xmetrics(const orc_case& cs, const Mesh& m, std::vector<XM>& X)
 {
     const int ni = m.ni, nj = m.nj;
@@ -1077,7 +1077,7 @@ void xmetrics(const orc_case& cs, const Mesh& m, std::vector<XM>& X)
     }
 }
 
-// spatial/lsqMetrics.h:269-301, lsq/qmetric.ipp:367-389; QMetric stored [nvar][2]
+// spatial/lsqMetrics.h:181-213, lsq/qmetric.ipp:48-70; QMetric stored [nvar][2]
 void qmetrics(const orc_case& cs, const Mesh& m, const double* q, std::vector<double>& Q)
 {
     const int ni = m.ni, nj = m.nj, nv = nvar_of(cs.law);
@@ -1133,7 +1133,7 @@ FluxRes muscl_flux(const orc_case& cs, const Species& sp, const Face& f, const C
     const double s1 = 1. / (dl + dr);
     double dqdx_c[MAXV];
     for (int v = 0; v < nv; ++v) dqdx_c[v] = dq_c[v] * s1;
-    // lsq::bias_solves, lsq/factor_solve.ipp:461-490
+    // lsq::bias_solves, lsq/factor_solve.ipp:71-100
     const XM dxm_c = xmetric(dx_c);
     double xbl[3], xbr[3];
     for (int k = 0; k < 3; ++k) {
@@ -1249,7 +1249,7 @@ void residual_calc(Field& F, const double* q, std::vector<FluxRes>& res)
     for (int b = 0; b < 4; ++b) {
         const int type = cs.bc[b];
         if (type == ORC_BC_PERIODIC) {
-            if (b == 0) {   // periodicLoop.h:277-300
+            if (b == 0) {   // periodicLoop.h:129-152
                 for (int j = 0; j < nj; ++j) {
                     const Face f = surface(m.node(0, j), m.node(0, j + 1));
                     const Cell l = image_of(m.cell(ni - 1, j), cs.period_i);
@@ -1258,7 +1258,7 @@ void residual_calc(Field& F, const double* q, std::vector<FluxRes>& res)
                     acc_minus(Rc(ni - 1, j), fr, nv);
                     acc_plus(Rc(0, j), fr, nv);
                 }
-            } else if (b == 2) {   // periodicLoop.h:301-324
+            } else if (b == 2) {   // periodicLoop.h:153-176
                 for (int i = 0; i < ni; ++i) {
                     const Face f = surface(m.node(i + 1, 0), m.node(i, 0));
                     const Cell l = image_of(m.cell(i, nj - 1), cs.period_j);
@@ -1300,7 +1300,7 @@ void residual_calc(Field& F, const double* q, std::vector<FluxRes>& res)
     }
 }
 
-// spatial/spectralRadius.h:385-414 (transform_reduce max, init numeric_limits::min())
+// spatial/spectralRadius.h:19-48 (transform_reduce max, init numeric_limits::min())
 double spectral_radius_field(const Mesh& m, const std::vector<FluxRes>& res)
 {
     double r = std::numeric_limits<double>::min();
@@ -1371,7 +1371,7 @@ void orc_dual(int ni, int nj, const double* nodes, double* cells)
     }
 }
 
-// mesh/generate/twoD.h:178-203
+// mesh/generate/twoD.h:15-40
 void orc_mesh_linspace(int ni, int nj, double lox, double hix, double loy, double hiy, double* nodes)
 {
     const double dx = (hix - lox) / (size_t)ni;
@@ -1383,7 +1383,7 @@ void orc_mesh_linspace(int ni, int nj, double lox, double hix, double loy, doubl
         }
 }
 
-// mesh/generate/twoD.h:302-342
+// mesh/generate/twoD.h:139-179
 void orc_mesh_cylinder(int nr, int nth, double rc, double rb, double* nodes)
 {
     const double dr = (rb - rc) / (size_t)nr;
@@ -1399,7 +1399,7 @@ void orc_mesh_cylinder(int nr, int nth, double rc, double rb, double* nodes)
 
 /*
  * Geometric ratio s with dr0*(s^nr-1)/(s-1) = rb-rc (bisection to the last bit).
- * New generator next to make_cylinder_mesh (twoD.h:302-342): the O-grid of the
+ * New generator next to make_cylinder_mesh (twoD.h:139-179): the O-grid of the
  * cylinder cases with geometric radial stretching.
  */
 double orc_ogrid_ratio(int nr, double rc, double rb, double dr0)
@@ -1598,7 +1598,7 @@ double orc_spectral_radius(int ni, int nj, const double* cells, const double* re
     return r;
 }
 
-// timestepping/rungeKutta.h:37-167 and :172-197, spatial/eulerForwardUpdate.h:236-290
+// timestepping/rungeKutta.h:37-167 and :172-197, spatial/eulerForwardUpdate.h:38-92
 int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, double* q0,
                   const double* bref, int nsteps, double* t_out, double* timers_ms)
 {

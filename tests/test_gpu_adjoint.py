@@ -183,3 +183,24 @@ def test_torch_autograd_gradient_through_mach_number():
     print("dJ/dM autograd %.12e fd %.12e" % (M.grad.item(), fd))
     assert abs(J.item() - J_oracle(0.4)) <= 1e-10 * abs(J_oracle(0.4)) + 1e-15
     assert abs(M.grad.item() - fd) <= 1e-6 * abs(fd)
+
+
+def test_backward_after_a_second_forward_on_the_same_solver_raises():
+    """The context keeps the checkpoints of its latest forward pass only: the backward of an
+    earlier output must refuse rather than use the later pass's checkpoints."""
+    import torch
+    from hyperboliconvergence_amd.adjoint import TrajectorySolver, trajectory
+    cfg = _cfg(nsteps=4)
+    q0 = torch.tensor(cfg["q"], dtype=torch.float64, requires_grad=True)
+    bref = torch.tensor(cfg["bref"], dtype=torch.float64)
+    M = torch.tensor(cfg["case"].sp.minf, dtype=torch.float64)
+    solver = TrajectorySolver(cfg["nodes"], cfg["case"], 4, 2)
+    try:
+        first = trajectory(solver, q0, bref, M)
+        second = trajectory(solver, q0 * 1.0001, bref, M)
+        with pytest.raises(RuntimeError, match="another forward pass"):
+            first.sum().backward()
+        second.sum().backward()   # the latest output still differentiates
+        assert torch.isfinite(q0.grad).all()
+    finally:
+        solver.close()

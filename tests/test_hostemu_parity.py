@@ -117,3 +117,16 @@ def test_product_geometry_matches_oracle():
     cells, X, fi, fj = pyemu.geometry(cs, nodes)
     assert np.array_equal(cells, po.dual(nodes), equal_nan=True)
     assert np.array_equal(X, po.xmetrics(cs, po.dual(nodes)), equal_nan=True)
+
+
+@pytest.mark.parametrize("flux", [po.FLUX_SLAU, po.FLUX_AUSMPUP])
+def test_adaptive_low_mach_scaling_bit_identical(flux):
+    """LowMachScaling::Adaptive with a finite species dt (mu = lref / (a dt) ~ 0.4)."""
+    cs, nodes, q, bref = _ogrid(9, 20, 1, flux, po.LIM_VANALBADA2)
+    cs.vscale = cs.pscale = po.SCALE_ADAPTIVE
+    cs.sp.dt = 2.0
+    r, _ = pyemu.residual(cs, nodes, q, bref)
+    assert np.array_equal(r, po.residual(cs, nodes, q, bref))
+    qe, _, _ = pyemu.integrate(cs, nodes, q, bref, 5)
+    qo, _, _ = po.integrate(cs, nodes, q, 5, bref)
+    assert np.array_equal(qe, qo)

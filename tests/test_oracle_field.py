@@ -28,7 +28,7 @@ def scalar_case(limiter, period_i, period_j, flux=po.FLUX_RUSANOV):
 
 @pytest.mark.parametrize("mesh", ["cartesian", "ogrid"])
 def test_lsq_exact_for_linear_fields(mesh):
-    """Biased LSQ gradients (lsq/factor_solve.ipp:461-490) of a linear field are exact on interior
+    """Biased LSQ gradients (lsq/factor_solve.ipp:71-100) of a linear field are exact on interior
     faces of any mesh: g_L.n = g_R.n = grad(q).n."""
     if mesh == "cartesian":
         nodes = po.mesh_linspace(9, 7, 0.0, 1.0, 0.0, 1.3)
@@ -57,7 +57,7 @@ def test_lsq_exact_for_linear_fields(mesh):
 
 
 def test_xmetric_mirror_points_on_nonperiodic_sides():
-    """lsqMetrics.h:194-251 adds the mirror point c0 + (c0 - c1) on non-periodic sides, so a corner
+    """lsqMetrics.h:104-163 adds the mirror point c0 + (c0 - c1) on non-periodic sides, so a corner
     cell of a uniform Cartesian mesh has X = diag(2h^2, 2h^2) like an interior cell."""
     nodes = po.mesh_linspace(5, 5, 0.0, 1.0, 0.0, 1.0)
     cells = po.dual(nodes)
@@ -117,7 +117,7 @@ def test_residual_conservation_and_freestream_periodic():
 def test_freestream_preserved_on_closed_ogrid_with_ghost_boundaries():
     """Uniform state with Fixed ghost values equal to it: every face flux is the exact flux and
     the face normals of each quadrilateral close, so R = 0 to round-off.  Pins the orientation and
-    sign of every boundary loop (ghostCellLoop.h:133-224, periodicLoop.h:277-324)."""
+    sign of every boundary loop (ghostCellLoop.h:133-224, periodicLoop.h:129-176)."""
     nodes = po.mesh_ogrid(12, 24, 0.5, 20.0, 1e-2)
     qinf = np.array([0.3, 0.1, 1.0, 1.0 / G])
     for flux in range(5):
@@ -197,7 +197,7 @@ def _advect_error(n, lim, nsteps_per_n):
     q[..., 2] = f0(x, y)
     nsteps = nsteps_per_n * n // 32
     qf, t, _ = po.integrate(cs, nodes, q, nsteps)
-    tp = 4.0 * t   # physical time: the reference's volume is area/4 (geometry/volume.ipp:87-88)
+    tp = 4.0 * t   # physical time: the reference's volume is area/4 (geometry/volume.ipp:57-58)
     exact = f0((x - 1.0 * tp) % 1.0, (y - 0.5 * tp) % 1.0)
     return np.max(np.abs(qf[..., 2] - exact)), tp
 
@@ -296,7 +296,7 @@ def test_subsonic_isentropic_vortex_converges(flux):
 
 
 def test_supersonic_faces_double_the_pressure_flux():
-    """machSplittings.ipp:197-202 (eps = numeric_limits::max()) makes P1 identically 1, so on a face
+    """machSplittings.ipp:26-31 (eps = numeric_limits::max()) makes P1 identically 1, so on a face
     supersonic on both sides AUSM+up's pressure flux is (p_L + p_R - p_u) A instead of the upwind p_L A
     (ausmPlusUP.ipp:116-127).  This is why the strength-5 vortex of the benchmark, locally supersonic
     near its core, does not converge with AUSM+up or SLAU."""

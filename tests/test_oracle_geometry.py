@@ -1,7 +1,7 @@
 """Pins of the oracle's geometry, mesh and Runge-Kutta tables by closed forms.
 
-geom::surface (geometry/surface.ipp:12-29), geom::volume (geometry/volume.ipp:55-96),
-dual (mesh/dual.ipp:121-162), make_linspace_mesh (mesh/generate/twoD.h:178-203),
+geom::surface (geometry/surface.ipp:12-29), geom::volume (geometry/volume.ipp:25-66),
+dual (mesh/dual.ipp:42-83), make_linspace_mesh (mesh/generate/twoD.h:15-40),
 the stretched O-grid, and ODE::Explicit tables (ode.h:107-197) + classical RK4.
 """
 import math

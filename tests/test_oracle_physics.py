@@ -223,7 +223,7 @@ def test_roe_unprec_ws_perturbations_scale_with_the_free_stream_mach_number():
 def test_roe_rest_states_rotate_and_frame_mixing_quirk():
     """States at rest: the face-frame rotation is exact on any face.  With velocity, the
     reference's Roe dissipation keeps Roe-average velocities in the global frame while the jump
-    is in the face frame (base/fluxes/roeFlux.ipp:84-91): on a y-face the result differs from the
+    is in the face frame (base/fluxes/roeFlux.ipp:40-54): on a y-face the result differs from the
     rotated x-face result, which this test requires."""
     cs = euler_case(flux=po.FLUX_ROE)
     ql, qr = prim_state(0, 0, 1.0, 1.0), prim_state(0, 0, 0.8, 0.7)
@@ -241,7 +241,7 @@ def test_roe_rest_states_rotate_and_frame_mixing_quirk():
 
 
 def test_slau_always_left_upwinding_quirk():
-    """slau.ipp:251-267: the switch is (ua > 0) with ua = sqrt(0.5(|u_L|^2+|u_R|^2)), so the
+    """slau.ipp:18, 123-140: the switch is (ua > 0) with ua = sqrt(0.5(|u_L|^2+|u_R|^2)), so the
     transported velocity and enthalpy are always the left state's, whatever the sign of mdot."""
     cs = euler_case(flux=po.FLUX_SLAU)
     for ul in (0.3, -0.4):
@@ -279,7 +279,7 @@ def test_mach_splittings():
         assert math.isclose(po.machsplit(2, 1, M), 0.25 * (M + 1) ** 2, rel_tol=1e-15)
     for M in (-3.0, -1.5, 1.2, 2.0):
         assert po.machsplit(1, 1, M) + po.machsplit(1, -1, M) == M
-        # machSplittings.ipp:197-202: eps = numeric_limits::max() makes P1 identically 1
+        # machSplittings.ipp:26-31: eps = numeric_limits::max() makes P1 identically 1
         assert po.machsplit(11, 1, M) == 1.0 and po.machsplit(11, -1, M) == 1.0
 
 
@@ -301,10 +301,10 @@ def test_limiter_closed_forms():
     assert L(po.LIM_VANALBADA2, 2.0, 2.0) == 2.0 and L(po.LIM_VANALBADA2, 1.0, -1.0) == 0.0
     assert math.isclose(L(po.LIM_VANALBADA2, 1.0, 3.0), (3 + 9) / 10, rel_tol=1e-15)
     assert L(po.LIM_SMOOTHVANALBADA, 1.0, -1.0) == 0.0 and math.isclose(L(po.LIM_SMOOTHVANALBADA, 1.0, -2.0), (-2 + 4) / 5)
-    # VanLeer2 (limiter.ipp:216-224) returns the ratio (b+|b|)/(a+|b|), not a slope
+    # VanLeer2 (limiter.ipp:81-89) returns the ratio (b+|b|)/(a+|b|), not a slope
     assert L(po.LIM_VANLEER2, 1.0, 1.0) == 1.0 and L(po.LIM_VANLEER2, 2.0, 2.0) == 1.0
     assert L(po.LIM_VANLEER2, 1.0, -1.0) == 0.0 and math.isclose(L(po.LIM_VANLEER2, 1.0, 3.0), 1.5)
-    # Cada3 branches (limiter.ipp:237-293)
+    # Cada3 branches (limiter.ipp:102-158)
     assert L(po.LIM_CADA3, 1.0, -3.0) == 0.0
     assert math.isclose(L(po.LIM_CADA3, 1.0, -0.5), 0.5)
     assert L(po.LIM_CADA3, 1.0, -0.2) == 0.4

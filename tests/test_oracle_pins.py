@@ -1,0 +1,351 @@
+"""Closed-form pins of the oracle's flux and boundary functions at special states.
+
+Each test derives the expected flux by hand from the published scheme at a state where most
+terms vanish, so that one coefficient of the reference's algorithm decides the result (a wrong
+coefficient, sign, switch or frame fails the test):
+
+* AUSM+up (euler/fluxes/ausmPlusUP.ipp:6-171, Liou 2006 with the reference's constants):
+  a symmetric normal-velocity jump at rest leaves only the pressure diffusion Ku P+ P- rho fa a dU
+  (Ku = 1.5, alpha0 = 3/16 through P5, fa of the clamped Mach number, quirk A-Q6); a pressure
+  jump at rest leaves only the mass diffusion -Kp dp / (fa rho a) (Kp = 0.25, the 1/fa(minf)
+  law) and the spectral-radius scaling lmd = (m0 + 1) / (2 fa) (ausmPlusUP.ipp:157-168); a
+  pressure jump in uniform flow pins sigma; the Adaptive scaling takes m0 = lref / (a dt).
+* SLAU / SLAU2 (euler/fluxes/slau.ipp:4-161, Shima & Kitamura 2011): a pressure jump at rest
+  and in uniform flow (chi = (1 - min(1, M))^2, the 1/a scaling of the pressure term); a contact
+  with unequal speeds (the density-weighted |u_n|); a velocity jump (SLAU2's -M (P+ + P- - 1)
+  (p_L + p_R)); a supersonic symmetric expansion (the switch g makes the mass flux vanish); the
+  low-Mach scaling template arguments have no effect (slau.ipp:37-64 computes them unused).
+* Roe on an oblique face (base/fluxes/roeFlux.ipp:22-58, quirk A-Q2): the reference rotates the
+  conserved jump into the face frame but converts it about the Roe state with global-frame
+  velocities, so its dissipation is R^T V diag|lambda_n| V^-1 R dq where V holds the
+  eigenvectors of the x-direction flux Jacobian at the global-frame Roe state and lambda_n the
+  face-normal wave speeds.  The Jacobian is built here by complex-step differentiation of the
+  textbook Euler flux and the matrix function by Sylvester's formula, independent of the
+  oracle's transforms; the face-frame variant must equal the textbook |A_n| instead.
+* Entropy far-field boundary (euler/boundary/entropy.ipp:18-88): subsonic inflow takes the
+  far-field velocity and entropy with the interior pressure, supersonic inflow the far-field
+  state; the signed transverse-flow test returns the interior state for every outflow.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+G = 1.4
+ASC = 2.0 * (G - 1.0) / (G + 1.0)
+
+
+def case(flux, minf=0.3, vscale=None, pscale=None, dt=None, basis=po.BASIS_PRIMITIVE):
+    cs = po.make_case(law=po.LAW_EULER, basis=basis, flux=flux, minf=minf, vscale=vscale, pscale=pscale)
+    if dt is not None:
+        cs.sp.dt = dt
+    return cs
+
+
+def face(n, area=1.0):
+    n = np.asarray(n, float) / np.linalg.norm(n)
+    return po.face_from(area, (0.0, 0.0), n)
+
+
+def P(u, v, r, p):
+    return np.array([u, v, r, p], float)
+
+
+def H(q):
+    u, v, r, p = q
+    return G / (G - 1.0) * p / r + 0.5 * (u * u + v * v)
+
+
+def P3(sign, m):
+    """Liou's third-order pressure polynomials P3+(M) = (M+1)^2 (2-M)/4, P3-(M) = (M-1)^2 (2+M)/4."""
+    return 0.25 * (m + 1) ** 2 * (2 - m) if sign > 0 else 0.25 * (m - 1) ** 2 * (2 + m)
+
+
+# ---------------- AUSM+up ----------------
+
+@pytest.mark.parametrize("delta,minf", [(0.01, 0.3), (0.2, 0.05), (0.5, 0.3)])
+def test_ausm_velocity_jump_at_rest_is_pressure_diffusion_with_Ku_1p5(delta, minf):
+    r, p, A = 1.1, 0.9, 0.7
+    ql, qr = P(-delta, 0, r, p), P(delta, 0, r, p)
+    F = po.flux(case(po.FLUX_AUSMPUP, minf), face((1, 0), A), ql, qr)
+    a_s = math.sqrt(ASC * H(ql))           # both critical speeds are sqrt(ascoeff H): the flow diverges
+    m = delta / a_s
+    m0 = min(max(m, minf), 1.0)             # Convective scaling: clamp(m0, minf, 1)
+    fa = m0 * (2 - m0)
+    alpha = 0.1875 * (5 * fa * fa - 4)
+    Pm = 0.25 * (1 - m) ** 2 * (2 + m) - alpha * m * (1 - m * m) ** 2   # P5+(-m) = P5-(m)
+    Ku = 1.5
+    pu = Ku * Pm * Pm * r * fa * a_s * (2 * delta)
+    assert math.isclose(F[0], (2 * Pm * p - pu) * A, rel_tol=1e-13)
+    assert abs(F[1]) == 0.0
+    assert abs(F[2]) <= 1e-15 and abs(F[3]) <= 1e-14          # M4+(-m) + M4-(m) = 0: no mass flux
+    lmd = 0.5 * (m0 + 1) / fa
+    amax = math.sqrt(max(a_s * a_s, G * p / r))
+    assert math.isclose(F[4], (lmd * amax + delta) * A, rel_tol=1e-13, abs_tol=1e-13)
+    # the textbook Ku = 0.75 would move the momentum flux by this much
+    assert abs(F[0] - (2 * Pm * p - 0.5 * pu) * A) > 1e3 * abs(F[0]) * 1e-13
+
+
+@pytest.mark.parametrize("pscale,dt,minf", [(po.SCALE_CONVECTIVE, None, 0.3), (po.SCALE_CONVECTIVE, None, 0.02),
+                                            (po.SCALE_ADAPTIVE, "mu=0.4", 0.1), (po.SCALE_ADAPTIVE, "mu=0.01", 0.1),
+                                            (po.SCALE_ACOUSTIC, None, 0.3)])
+def test_ausm_pressure_jump_at_rest_is_mass_diffusion_with_Kp_and_fa(pscale, dt, minf):
+    r, pl, pr, A = 1.0, 1.0, 1.2, 1.3
+    ql, qr = P(0, 0, r, pl), P(0, 0, r, pr)
+    a_s = min(math.sqrt(ASC * H(ql)), math.sqrt(ASC * H(qr)))
+    mu_target = None
+    if dt is not None:
+        mu_target = float(dt.split("=")[1])
+        dt = 1.0 / (a_s * mu_target)        # lref = 1: mu = lref / (a dt)
+    F = po.flux(case(po.FLUX_AUSMPUP, minf, pscale=pscale, dt=dt), face((1, 0), A), ql, qr)
+    if pscale == po.SCALE_CONVECTIVE:
+        m0p = minf                                   # m0 = 0 clamped to minf
+    elif pscale == po.SCALE_ACOUSTIC:
+        m0p = 1.0
+    else:
+        m0p = min(max(mu_target, minf), 1.0)         # Adaptive: clamp(max(m0, mu), minf, 1)
+    fa = m0p * (2 - m0p)
+    Kp = 0.25
+    ua = -a_s * Kp * (pr - pl) / (fa * r * a_s * a_s) * A   # M4+(0) + M4-(0) = 0
+    assert math.isclose(F[2], ua * r, rel_tol=1e-12)
+    assert math.isclose(F[3], ua * r * H(qr if ua < 0 else ql), rel_tol=1e-12)
+    assert math.isclose(F[0], 0.5 * (pl + pr) * A, rel_tol=1e-13)      # P5+(0) = P5-(0) = 1/2
+    assert F[1] == 0.0
+    lmd = 0.5 * (m0p + 1) / fa
+    amax = math.sqrt(max(a_s * a_s, G * pl / r, G * pr / r))
+    assert math.isclose(F[4], (lmd * amax + abs(ua)) * A, rel_tol=1e-12)   # |ua| already holds the area
+
+
+def test_ausm_pressure_jump_in_uniform_flow_pins_sigma():
+    u, rl, rr, pl, pr, A = 0.3, 1.0, 1.0, 1.0, 1.1, 0.8
+    ql, qr = P(u, 0.1, rl, pl), P(u, -0.05, rr, pr)
+    F = po.flux(case(po.FLUX_AUSMPUP, 0.05), face((1, 0), A), ql, qr)
+    a_s = min(math.sqrt(ASC * H(ql)), math.sqrt(ASC * H(qr)))
+    m = u / a_s
+    fa = m * (2 - m)
+    sigma, Kp = 1.0, 0.25
+    mp = Kp * max(1 - sigma * m * m, 0.0) * (pr - pl) / (fa * 0.5 * (rl + rr) * a_s * a_s)
+    ua = a_s * (m - mp) * A                  # M4+(m) + M4-(m) = m exactly
+    assert math.isclose(F[2], ua * rl, rel_tol=1e-12)
+    assert math.isclose(F[1], ua * rl * 0.1, rel_tol=1e-12)   # upwind tangential velocity
+
+
+# ---------------- SLAU / SLAU2 ----------------
+
+def _a(q):
+    return math.sqrt(G * q[3] / q[2])
+
+
+def test_slau_pressure_jump_at_rest():
+    ql, qr, A = P(0, 0, 1.0, 1.0), P(0, 0, 1.0, 1.3), 0.6
+    F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), ql, qr)
+    aa = 0.5 * (_a(ql) + _a(qr))
+    mdot = -0.5 * (0.3 / aa) * A              # chi = 1 at rest
+    assert math.isclose(F[2], mdot, rel_tol=1e-13)
+    assert math.isclose(F[3], mdot * H(qr), rel_tol=1e-13)    # ua = 0: the right state is upwind
+    assert math.isclose(F[0], 0.5 * 2.3 * A, rel_tol=1e-13)   # P3+(0) = P3-(0) = 1/2
+    assert math.isclose(F[4], (_a(qr) + 0.0) * A, rel_tol=1e-13)
+
+
+@pytest.mark.parametrize("u", [0.2, 0.7])
+def test_slau_pressure_jump_in_uniform_flow_pins_chi_and_slau2(u):
+    r, pl, pr, A = 1.0, 1.0, 1.15, 0.9
+    ql, qr = P(u, 0, r, pl), P(u, 0, r, pr)
+    F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), ql, qr)
+    aa = 0.5 * (_a(ql) + _a(qr))
+    ma = u / aa
+    chi = (1 - min(1.0, ma)) ** 2
+    dp = pr - pl
+    mdot = (r * u - 0.5 * chi * dp / aa) * A
+    assert math.isclose(F[2], mdot, rel_tol=1e-12)
+    bl, br = P3(1, u / aa), P3(-1, u / aa)
+    pa = 0.5 * ((pl + pr) - (bl - br) * dp + ma * (bl + br - 1) * (pl + pr)) * A
+    assert math.isclose(F[0], mdot * u + pa, rel_tol=1e-12)
+
+
+def test_slau_contact_mass_flux_weights_speeds_by_density():
+    rl, rr, p, A = 0.8, 1.6, 1.0, 1.1
+    u1, u2 = 0.3, 0.1
+    F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), P(u1, 0, rl, p), P(u2, 0, rr, p))
+    # dp = 0, g = 0:  mdot = (rl u1 + rr u2 - una (rr - rl)) / 2 with una = (rl u1 + rr u2) / (rl + rr)
+    assert math.isclose(F[2], (rl * u1 + rr * u2) * rl / (rl + rr) * A, rel_tol=1e-13)
+    for u in (0.25, -0.25):   # equal speeds: exact upwinding of the mass
+        F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), P(u, 0, rl, p), P(u, 0, rr, p))
+        assert math.isclose(F[2], (rl if u > 0 else rr) * u * A, rel_tol=1e-13)
+
+
+def test_slau2_velocity_jump_pressure_term():
+    r, p, d, A = 1.0, 1.0 / G, 0.2, 1.0
+    ql, qr = P(-d, 0, r, p), P(d, 0, r, p)
+    F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), ql, qr)
+    a = _a(ql)
+    m = d / a
+    B = P3(1, -m)
+    assert math.isclose(B, P3(-1, m), rel_tol=1e-15)
+    # SLAU2: p_du = -M (P+ + P- - 1)(p_L + p_R); SLAU1 would take (1 - chi) instead of M
+    assert math.isclose(F[0], -0.0 * d + p * (1 + m * (2 * B - 1)) * A, rel_tol=1e-13, abs_tol=1e-15)
+    assert abs(F[2]) <= 1e-16
+
+
+def test_slau_supersonic_expansion_switch_g_cancels_the_mass_flux():
+    rl, rr, p, U, A = 1.0, 0.5, 1.0, 2.5, 1.0
+    ql, qr = P(-U, 0, rl, p), P(U, 0, rr, p)
+    F = po.flux(case(po.FLUX_SLAU), face((1, 0), A), ql, qr)
+    aa = 0.5 * (_a(ql) + _a(qr))
+    assert U / aa > 1.0
+    assert F[2] == 0.0
+    ma = U / aa   # |M| > 1: P1 = 1 on both sides (quirk A-Q5)
+    assert math.isclose(F[0], p * (1 + ma) * A, rel_tol=1e-13)
+
+
+def test_slau_low_mach_scaling_arguments_have_no_effect():
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        ql = P(*rng.uniform(-0.5, 0.5, 2), rng.uniform(0.5, 2), rng.uniform(0.5, 1.5))
+        qr = P(*rng.uniform(-0.5, 0.5, 2), rng.uniform(0.5, 2), rng.uniform(0.5, 1.5))
+        f = face(rng.standard_normal(2), 0.7)
+        ref = po.flux(case(po.FLUX_SLAU, 0.2, po.SCALE_CONVECTIVE, po.SCALE_ACOUSTIC), f, ql, qr)
+        for vs in range(3):
+            for ps in range(3):
+                for dt in (None, 0.01):
+                    F = po.flux(case(po.FLUX_SLAU, 0.2, vs, ps, dt), f, ql, qr)
+                    assert np.array_equal(F, ref)
+
+
+# ---------------- Roe on oblique faces ----------------
+
+def _cons(q):
+    u, v, r, p = q
+    return np.array([r * u, r * v, r, p / (G - 1) + 0.5 * r * (u * u + v * v)])
+
+
+def _flux_x(qc):
+    m, n, r, E = qc
+    p = (G - 1) * (E - 0.5 * (m * m + n * n) / r)
+    return np.array([m * m / r + p, m * n / r, m, (E + p) * m / r])
+
+
+def _jacobian_x(qc):
+    """dF_x/dq by complex-step differentiation (exact to round-off)."""
+    h = 1e-30
+    J = np.zeros((4, 4))
+    for k in range(4):
+        e = np.zeros(4, complex)
+        e[k] = 1j * h
+        J[:, k] = np.imag(_flux_x(qc.astype(complex) + e)) / h
+    return J
+
+
+def _roe_state(ql, qr):
+    sl, sr = math.sqrt(ql[2]), math.sqrt(qr[2])
+    w = np.array([sl, sr]) / (sl + sr)
+    u = w[0] * ql[0] + w[1] * qr[0]
+    v = w[0] * ql[1] + w[1] * qr[1]
+    h = w[0] * H(ql) + w[1] * H(qr)
+    a = math.sqrt((G - 1) * (h - 0.5 * (u * u + v * v)))
+    return u, v, sl * sr, h, a
+
+
+def _abs_by_families(J, ux, a, lam):
+    """Sylvester's formula: sum over the families (ux-a, ux, ux+a) of J of f_k times the spectral
+    projector, with f the face-normal speeds |u_n - a|, |u_n|, |u_n + a|."""
+    I = np.eye(4)
+    ev = (ux - a, ux, ux + a)
+    out = np.zeros((4, 4))
+    for k in range(3):
+        Pk = I.copy()
+        for j in range(3):
+            if j != k:
+                Pk = Pk @ (J - ev[j] * I) / (ev[k] - ev[j])
+        out += lam[k] * Pk
+    return out
+
+
+def _rot(n, t):
+    R = np.eye(4)
+    R[0, :2] = n
+    R[1, :2] = t
+    return R
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_roe_oblique_face_dissipation_mixes_frames_as_the_reference(seed):
+    rng = np.random.default_rng(seed)
+    ql = P(rng.uniform(-0.4, 0.4), rng.uniform(-0.4, 0.4), rng.uniform(0.8, 1.2), rng.uniform(0.6, 0.9))
+    qr = ql + np.array([*rng.uniform(-0.05, 0.05, 2), rng.uniform(-0.05, 0.05), rng.uniform(-0.05, 0.05)])
+    th = rng.uniform(0, 2 * math.pi)
+    n, A = np.array([math.cos(th), math.sin(th)]), rng.uniform(0.5, 2.0)
+    f = po.face_from(A, (0.0, 0.0), n)
+    t = np.array([f.tx, f.ty])
+    u, v, r, h, a = _roe_state(ql, qr)
+    un = u * n[0] + v * n[1]
+    lam = (abs(un - a), abs(un), abs(un + a))
+    assert min(lam) > 0.02   # away from sonic and stagnation points: the entropy fix is inactive
+    R = _rot(n, t)
+    dq = _cons(qr) - _cons(ql)
+    central = po.flux(case(po.FLUX_CENTRAL), f, ql, qr)
+
+    # the reference: global-frame Roe velocities in the x-direction eigensystem, face-normal speeds
+    qroe_g = np.array([r * u, r * v, r, r * h - (r * a * a / G)])   # rho E = rho H - p, p = rho a^2 / gamma
+    Jg = _jacobian_x(qroe_g)
+    d_ref = -0.5 * A * (R.T @ (_abs_by_families(Jg, u, a, lam) @ (R @ dq)))
+    F = po.flux(case(po.FLUX_ROE), f, ql, qr)
+    assert np.allclose(F[:4] - central[:4], d_ref, rtol=1e-9, atol=1e-13)
+    lam_roe = (abs(un) + a) * A
+    assert math.isclose(F[4], max(central[4], lam_roe), rel_tol=1e-13)
+
+    # the face-frame variant: the textbook |A_n| dq (Roe velocity rotated into the face frame)
+    qroe_f = np.array([r * un, r * (u * t[0] + v * t[1]), r, qroe_g[3]])
+    Jf = _jacobian_x(qroe_f)
+    d_ff = -0.5 * A * (R.T @ (_abs_by_families(Jf, un, a, lam) @ (R @ dq)))
+    Fff = po.flux(case(po.FLUX_ROE_FACEFRAME), f, ql, qr)
+    assert np.allclose(Fff[:4] - central[:4], d_ff, rtol=1e-9, atol=1e-13)
+    # the two differ unless the face is x-aligned
+    if abs(n[1]) > 0.1 and abs(v) + abs(u) > 0.05:
+        assert not np.allclose(d_ref, d_ff, rtol=1e-6, atol=1e-10)
+
+
+# ---------------- entropy far-field boundary ----------------
+
+def _sb(q):
+    return q[3] / q[2] ** G   # a^2 / (gamma rho^(gamma-1)) = p / rho^gamma
+
+
+@pytest.mark.parametrize("basis", [po.BASIS_PRIMITIVE, po.BASIS_CONSERVED])
+@pytest.mark.parametrize("th", [0.0, 0.9, 2.5])
+def test_entropy_boundary_closed_forms(basis, th):
+    cs = case(po.FLUX_ROE, basis=basis)
+    n = np.array([math.cos(th), math.sin(th)])
+    f = po.face_from(1.0, (0.0, 0.0), n)
+    cell = [1.0, 0.5 * n[0], 0.5 * n[1]]    # the cell lies on the normal's side: n points into the domain
+    t = np.array([f.tx, f.ty])
+
+    def glob(un, ut, r, p):
+        vel = un * n + ut * t
+        return P(vel[0], vel[1], r, p)
+
+    def to_basis(q):
+        return q if basis == po.BASIS_PRIMITIVE else _cons(q)
+
+    def bc(qi, qf):
+        out = po.bc_update(cs, po.BC_ENTROPY, f, cell, to_basis(qi), to_basis(qf))
+        return out
+
+    # subsonic inflow: far-field velocity and entropy, interior pressure
+    qi, qf = glob(0.3, 0.05, 1.1, 0.8), glob(0.35, -0.1, 1.0, 1 / G)
+    ex = glob(0.35, -0.1, (0.8 / _sb(qf)) ** (1 / G), 0.8)
+    assert np.allclose(bc(qi, qf), to_basis(ex), rtol=1e-13, atol=1e-15)
+    # outflow: the transverse-flow test is signed (u_n / |u| < 1e-4, entropy.ipp:48-49), so every
+    # outflow returns the interior state and the subsonic-outflow branch is never reached
+    qi, qf = glob(-0.3, 0.05, 1.1, 0.8), glob(-0.35, -0.1, 1.0, 1 / G)
+    assert np.array_equal(bc(qi, qf), to_basis(qi))
+    # supersonic inflow / outflow: the upwind state
+    qi, qf = glob(2.0, 0.1, 1.0, 1 / G), glob(2.1, 0.0, 1.0, 1 / G)
+    assert np.allclose(bc(qi, qf), to_basis(qf), rtol=1e-14, atol=1e-15)
+    qi, qf = glob(-2.0, 0.1, 1.0, 1 / G), glob(-2.1, 0.0, 1.0, 1 / G)
+    assert np.array_equal(bc(qi, qf), to_basis(qi))
+    # transverse flow (normal Roe velocity below 1e-4 of its speed): the interior state
+    qi, qf = glob(0.0, 0.3, 1.0, 0.8), glob(0.0, 0.3, 1.0, 1 / G)
+    assert np.array_equal(bc(qi, qf), to_basis(qi))
