@@ -33,6 +33,7 @@ C_ABI_SYMBOLS = (
     "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
     "hc_set_tiles", "hc_get_tiles",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
+    "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
 )
 NTILECFG = 3
 
@@ -107,6 +108,11 @@ def library():
         "hc_adjoint_forward": (i, [P, i, i]),
         "hc_adjoint_backward": (i, [P, P, P, P, P]),
         "hc_tangent_steps": (i, [P, i, P, P, d, P]),
+        "hc_xmetrics": (i, [P, P]),
+        "hc_qmetrics": (i, [P, P, P]),
+        "hc_boundary_update": (i, [P, P, P, P]),
+        "hc_spectral_radius": (i, [P, P, P]),
+        "hc_forward_update": (i, [P, i, d, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -293,6 +299,44 @@ class Context:
         self._check(self._L.hc_tangent_steps(self._h, int(nsteps), _p(qdot0), bd, float(minf_dot), _p(out)),
                     "hc_tangent_steps")
         return out
+
+    # ---- spatial building blocks (single-case context) ----
+    def xmetrics(self):
+        """LSQ spatial metric per cell [ni][nj][3] = {xx, xy, yy}."""
+        out = np.zeros((self.ni, self.nj, 3))
+        self._check(self._L.hc_xmetrics(self._h, _p(out)), "hc_xmetrics")
+        return out
+
+    def qmetrics(self, q):
+        """LSQ solution metric [ni][nj][nv][2] of the state q (solution basis)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros((self.ni, self.nj, self.nv, 2))
+        self._check(self._L.hc_qmetrics(self._h, _p(q), _p(out)), "hc_qmetrics")
+        return out
+
+    def boundary_update(self, q, bref, ghost=None):
+        """Ghost values [rows][nv] of the non-periodic boundaries (periodic rows keep `ghost`)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        bref = np.ascontiguousarray(bref, dtype=np.float64)
+        out = np.zeros_like(bref) if ghost is None else np.array(ghost, dtype=np.float64, copy=True)
+        self._check(self._L.hc_boundary_update(self._h, _p(q), _p(bref), _p(out)), "hc_boundary_update")
+        return out
+
+    def spectral_radius(self, res):
+        """max over cells of lambda / volume of a residual [ni][nj][nv+1]."""
+        res = np.ascontiguousarray(res, dtype=np.float64)
+        out = C.c_double()
+        self._check(self._L.hc_spectral_radius(self._h, _p(res), C.byref(out)), "hc_spectral_radius")
+        return out.value
+
+    def forward_update(self, mode, coef, res, q0):
+        """mode 0: S(C(q0) + R coef / vol); mode 1: S(C(q0) + R coef / lambda)."""
+        res = np.ascontiguousarray(res, dtype=np.float64)
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        q1 = np.zeros_like(q0)
+        self._check(self._L.hc_forward_update(self._h, int(mode), float(coef), _p(res), _p(q0), _p(q1)),
+                    "hc_forward_update")
+        return q1
 
     def stream_triad(self, n, reps):
         g = C.c_double()

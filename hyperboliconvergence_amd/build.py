@@ -54,7 +54,7 @@ def _jobs(mixed_contract="fast", build_dir=None, waves_per_eu=(0, 0), mixed_fast
             cmd = [HIPCC, *COMMON, *fpflags, f"-DHC_LAW={law}", f"-DHC_BASIS={basis}", f"-DHC_FLUX={flux}",
                    f"-DHC_PREC={prec}", "-x", "hip", "-c", os.path.join(CSRC, "hc_kernel_inst.hip"), "-o", out]
             objs.append((out, cmd))
-    for src in ("hc_geom.cpp", "hc_capi.cpp", "hc_adjoint.cpp", "hc_adjoint_inst.hip"):
+    for src in ("hc_geom.cpp", "hc_capi.cpp", "hc_adjoint.cpp", "hc_adjoint_inst.hip", "hc_spatial.hip"):
         out = os.path.join(bdir, src.replace(".cpp", ".o").replace(".hip", ".o"))
         cmd = [HIPCC, *COMMON, "-ffp-contract=off", "-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
         objs.append((out, cmd))

@@ -1,21 +1,34 @@
 // hcamd.hpp -- C++20 drop-in for the reference's structured-mesh finite-volume hot path.
 //
 // Same names, template parameters and argument order as the reference's header-only API
-// (lacrymose/hyperboliConvergence cpp/inc/): flux functors, limiters, make_muscl_flux,
-// boundary-condition factories, Runge-Kutta tables, UnsteadyTimeControls, Mesh,
-// SolutionField, residualCalc and integrate.  The point-wise functors evaluate on the host
-// with the same code the HIP kernels run; residualCalc and integrate run on the GPU through
-// the C-ABI of include/hcamd.h.  Link with -lhcamd (hyperboliconvergence_amd/libhcamd.so).
+// (lacrymose/hyperboliConvergence cpp/inc/): the par:: containers and algorithms (par.hpp),
+// laws, variable sets and set2Set / set2State, flux functors, limiters, make_muscl_flux,
+// boundary-condition factories, Mesh with its node and cell arrays, SolutionField, the LSQ metric
+// types, Runge-Kutta tables, time controls, and the spatial and time-stepping entry points:
+// xmetrics, qmetrics, boundaryUpdate, residualCalc, spectralRadius, rungeKuttaAccumulation,
+// eulerForwardUpdateGlobal / Local and integrate.
+//
+// Where the work runs: residualCalc, boundaryUpdate and integrate run on the GPU through the
+// C-ABI of include/hcamd.h, on a device session kept per mesh and scheme (geometry is built once
+// and reused by later calls with the same mesh nodes, boundary types, species and scheme).  The
+// point functors and the per-cell maps whose reference signatures take host arrays only
+// (xmetrics, qmetrics, spectralRadius, rungeKuttaAccumulation, eulerForwardUpdate*) evaluate on
+// the host with the same point functions the kernels run (hc_point.hpp); the C-ABI offers device
+// versions of each of them.  Link with -lhcamd (hyperboliconvergence_amd/libhcamd.so).
 #pragma once
 
 #include "hcamd.h"
+#include "par.hpp"
 #include "../../csrc/hc_point.hpp"
 
 #include <array>
 #include <cassert>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
+#include <cstring>
 #include <limits>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -150,6 +163,21 @@ template <floating_point Real> Species<LawType::Euler, Real> get_air_species()
     return air;
 }
 
+
+// traits of a variable set (utils/type-traits.h: law_of_v, dim_of_v, fptype_of_t, vardelta_t)
+template <typename T> struct varset_traits;
+template <LawType L, int D, auto B, floating_point R> struct varset_traits<VariableSet<L, D, B, R>> {
+    static constexpr LawType law = L;
+    static constexpr int dim = D;
+    static constexpr auto basis = B;
+    using real = R;
+    using delta = VariableDelta<L, D, B, R>;
+};
+template <typename T> inline constexpr LawType law_of_v = varset_traits<T>::law;
+template <typename T> inline constexpr int dim_of_v = varset_traits<T>::dim;
+template <typename T> using fptype_of_t = typename varset_traits<T>::real;
+template <typename T> using vardelta_t = typename varset_traits<T>::delta;
+
 // ---------------- geometry (geometry/geometry.h, surface.ipp, volume.ipp) ----------------
 
 namespace geom {
@@ -200,55 +228,122 @@ template <floating_point Real> Surface<2, Real> surface(const Point<2, Real>& p0
 }
 }   // namespace geom
 
-// ---------------- structured mesh (mesh/mesh.h:19-57, generate/twoD.h) ----------------
 
-template <int nDim, floating_point Real> struct Mesh;
-
-template <floating_point Real> struct Mesh<2, Real> {
-    using Node = geom::Point<2, Real>;
-    std::size_t ni = 0, nj = 0;          // cells
-    std::vector<Node> nodes;             // (ni+1) x (nj+1), row-major
-    const Node& node(std::size_t i, std::size_t j) const { return nodes[i * (nj + 1) + j]; }
-    Node& node(std::size_t i, std::size_t j) { return nodes[i * (nj + 1) + j]; }
-    std::array<Real, 2> period_i{0, 0};   // translation to the periodic image across boundary 1
-    std::array<Real, 2> period_j{0, 0};   // ... across boundary 3
-};
-
-// mesh/generate/twoD.h:15-40
+namespace geom {
+// volume.ipp:25-66: the bilinear element's |J| and centre at the element centre (the volume is a
+// quarter of the area, the centre the vertex average)
 template <floating_point Real>
-Mesh<2, Real> make_linspace_mesh(const std::array<std::size_t, 2>& cellDims, Real lox, Real hix, Real loy, Real hiy)
+Volume<2, Real> volume(const Point<2, Real>& p0, const Point<2, Real>& p1, const Point<2, Real>& p2,
+                       const Point<2, Real>& p3)
 {
-    Mesh<2, Real> m;
-    m.ni = cellDims[0];
-    m.nj = cellDims[1];
-    m.nodes.resize((m.ni + 1) * (m.nj + 1));
-    const Real dx = (hix - lox) / m.ni, dy = (hiy - loy) / m.nj;
-    for (std::size_t i = 0; i <= m.ni; ++i)
-        for (std::size_t j = 0; j <= m.nj; ++j) m.node(i, j) = {{lox + i * dx, loy + j * dy}};
-    m.period_i = {hix - lox, 0};
-    m.period_j = {0, hiy - loy};
-    return m;
+    const Point<2, Real>* p[4] = {&p0, &p1, &p2, &p3};
+    const Real u = Real(0.5) * Real(0.5);
+    const Real du[4][2] = {{Real(-0.5) * Real(0.5), Real(0.5) * Real(-0.5)},
+                           {Real(0.5) * Real(0.5), Real(0.5) * Real(-0.5)},
+                           {Real(-0.5) * Real(0.5), Real(0.5) * Real(0.5)},
+                           {Real(0.5) * Real(0.5), Real(0.5) * Real(0.5)}};
+    Volume<2, Real> v;
+    for (int d = 0; d < 2; ++d) {
+        Real c = 0;
+        for (int k = 0; k < 4; ++k) c = c + u * (*p[k])[d];
+        v.centre[d] = c;
+    }
+    Real t[2][2];
+    for (int a = 0; a < 2; ++a)
+        for (int d = 0; d < 2; ++d) {
+            Real c = 0;
+            for (int k = 0; k < 4; ++k) c = c + du[k][a] * (*p[k])[d];
+            t[a][d] = c;
+        }
+    v.volume = std::fabs(t[0][0] * t[1][1] - t[0][1] * t[1][0]);
+    return v;
+}
+}   // namespace geom
+
+// ---------------- structured mesh (mesh/mesh.h:8-57, mesh/dual.ipp:42-83, generate/twoD.h) ----------------
+
+template <int nDim, floating_point Real> using MeshNodeArray = par::PrimalArray<geom::Point<nDim, Real>, nDim>;
+template <int nDim, floating_point Real> using MeshCellArray = par::DualArray<geom::Volume<nDim, Real>, nDim>;
+
+// cells of the dual mesh from the nodes (mesh/dual.ipp:42-83): corners {i,j} {i+1,j} {i,j+1} {i+1,j+1}
+template <floating_point Real> void dual(const MeshNodeArray<2, Real>& nodes, MeshCellArray<2, Real>& cells)
+{
+    const std::size_t ni = cells.shape(0), nj = cells.shape(1);
+    assert(nodes.shape(0) == ni + 1 && nodes.shape(1) == nj + 1);
+    for (std::size_t i = 0; i < ni; ++i)
+        for (std::size_t j = 0; j < nj; ++j)
+            cells({i, j}) = geom::volume(nodes({i, j}), nodes({i + 1, j}), nodes({i, j + 1}), nodes({i + 1, j + 1}));
+}
+template <floating_point Real> MeshCellArray<2, Real> dual(const MeshNodeArray<2, Real>& nodes)
+{
+    MeshCellArray<2, Real> cells(par::dualShape(nodes.shape()));
+    dual(nodes, cells);
+    return cells;
 }
 
-// mesh/generate/twoD.h:139-179 (uniform radial spacing, i = radius, j = theta)
+// node and cell arrays of a structured mesh, move-only (mesh/mesh.h:19-57).  period_i / period_j
+// are this build's addition: the translation from a cell to its periodic image across boundary 1
+// / 3, which periodic LSQ stencils need (zero for meshes that close on themselves, like an O-grid)
+template <int nDim, floating_point Real> struct Mesh {
+    using Node = geom::Point<nDim, Real>;
+    using Cell = geom::Volume<nDim, Real>;
+    using Face = geom::Surface<nDim, Real>;
+    using NodeArray = par::PrimalArray<Node, nDim>;
+    using CellArray = par::DualArray<Cell, nDim>;
+    using NodeShape = par::PrimalShape<nDim>;
+    using CellShape = par::DualShape<nDim>;
+
+    NodeShape node_shape;
+    CellShape cell_shape;
+    NodeArray nodes;
+    CellArray cells;
+    std::array<Real, 2> period_i{0, 0};
+    std::array<Real, 2> period_j{0, 0};
+
+    Mesh() = delete;
+    Mesh(const Mesh&) = delete;
+    Mesh(Mesh&&) = default;
+    Mesh& operator=(const Mesh&) = delete;
+    Mesh& operator=(Mesh&&) = default;
+    Mesh(const NodeShape& s) : node_shape(s), cell_shape(par::dualShape(s)), nodes(node_shape), cells(cell_shape) {}
+    Mesh(const CellShape& s) : node_shape(par::primalShape(s)), cell_shape(s), nodes(node_shape), cells(cell_shape) {}
+};
+
+// mesh/generate/twoD.h:15-40: [lox, hix] x [loy, hiy] with linear spacing (periodic translations set)
+template <floating_point Real>
+Mesh<2, Real> make_linspace_mesh(const par::DualShape2& cellDims, const Real lox, const Real hix, const Real loy,
+                                 const Real hiy)
+{
+    Mesh<2, Real> mesh(cellDims);
+    const std::size_t ni = cellDims[0], nj = cellDims[1];
+    const Real dx = (hix - lox) / ni, dy = (hiy - loy) / nj;
+    for (std::size_t i = 0; i <= ni; ++i)
+        for (std::size_t j = 0; j <= nj; ++j) mesh.nodes({i, j}) = {{lox + i * dx, loy + j * dy}};
+    dual(mesh.nodes, mesh.cells);
+    mesh.period_i = {hix - lox, 0};
+    mesh.period_j = {0, hiy - loy};
+    return mesh;
+}
+
+// mesh/generate/twoD.h:139-179: annulus rc..rb, uniform radial spacing, i = radius, j = theta
 template <floating_point Real>
 Mesh<2, Real> make_cylinder_mesh(const std::array<Real, 2> lengths, const std::array<std::size_t, 2> ncells)
 {
-    Mesh<2, Real> m;
-    m.ni = ncells[0];
-    m.nj = ncells[1];
-    m.nodes.resize((m.ni + 1) * (m.nj + 1));
-    const Real dr = (lengths[1] - lengths[0]) / m.ni;
-    const Real dth = (2. * M_PI) / m.nj;
-    for (std::size_t i = 0; i <= m.ni; ++i)
-        for (std::size_t j = 0; j <= m.nj; ++j) {
+    const std::size_t nr = ncells[0], nth = ncells[1];
+    Mesh<2, Real> mesh(par::DualShape2{{nr, nth}});
+    const Real dr = (lengths[1] - lengths[0]) / nr;
+    const Real dth = (2. * M_PI) / nth;
+    for (std::size_t i = 0; i <= nr; ++i)
+        for (std::size_t j = 0; j <= nth; ++j) {
             const Real r = lengths[0] + i * dr, th = j * dth;
-            m.node(i, j) = {{r * std::cos(th), r * std::sin(th)}};
+            mesh.nodes({i, j}) = {{r * std::cos(th), r * std::sin(th)}};
         }
-    return m;
+    dual(mesh.nodes, mesh.cells);
+    return mesh;
 }
 
-// O-grid with geometric radial stretching (first spacing dr0) and node nth identical to node 0
+// the O-grid of the cylinder cases (new, next to make_cylinder_mesh): geometric radial stretching
+// with first spacing dr0 reaching rb exactly, node nth identical to node 0
 template <floating_point Real>
 Mesh<2, Real> make_ogrid_mesh(const std::array<Real, 2> lengths, const std::array<std::size_t, 2> ncells, Real dr0)
 {
@@ -263,40 +358,71 @@ Mesh<2, Real> make_ogrid_mesh(const std::array<Real, 2> lengths, const std::arra
         (g(mid) < 0 ? lo : hi) = mid;
     }
     const Real s = 0.5 * (lo + hi);
-    Mesh<2, Real> m;
-    m.ni = nr;
-    m.nj = nth;
-    m.nodes.resize((nr + 1) * (nth + 1));
+    Mesh<2, Real> mesh(par::DualShape2{{nr, nth}});
     for (std::size_t i = 0; i <= nr; ++i) {
         const Real r = (i == nr) ? lengths[1] : lengths[0] + dr0 * (std::pow(s, Real(i)) - 1.) / (s - 1.);
         for (std::size_t j = 0; j <= nth; ++j) {
             const Real th = (2. * M_PI * Real(j == nth ? 0 : j)) / nth;
-            m.node(i, j) = {{r * std::cos(th), r * std::sin(th)}};
+            mesh.nodes({i, j}) = {{r * std::cos(th), r * std::sin(th)}};
         }
     }
-    return m;
+    dual(mesh.nodes, mesh.cells);
+    return mesh;
 }
 
-// ---------------- solution field (solutionField/solutionField.h:56-91) ----------------
+// ---------------- solution field (solutionField/solutionField.h:10-140) ----------------
 
+// interior values, and per boundary an array {n_along, 2}: (k, 0) the ghost value, (k, 1) the
+// reference value; move-only like the reference's
 template <typename VarSet, int nDim> struct SolutionField;
-
 template <typename VarSet> struct SolutionField<VarSet, 2> {
     static constexpr unsigned int nBoundaries = 4;
-    std::size_t ni, nj;
-    std::vector<VarSet> interior;                   // ni x nj, row-major
-    std::array<std::vector<VarSet>, 4> boundary;    // per boundary: n_along x 2 ([.,0] ghost, [.,1] reference)
-    std::array<int, 4> bcTypes{};                   // BoundaryType<Law> values
-    explicit SolutionField(const std::array<std::size_t, 2>& s) : ni(s[0]), nj(s[1]), interior(s[0] * s[1])
+    static constexpr LawType Law = law_of_v<VarSet>;
+    using VarDel = vardelta_t<VarSet>;
+    using VarField = par::DualArray<VarSet, 2>;
+    using BCType = BoundaryType<Law>;
+
+    VarField interior;
+    std::array<VarField, 4> boundary;
+    std::array<BCType, 4> bcTypes{};
+
+    SolutionField() = delete;
+    SolutionField(const SolutionField&) = delete;
+    SolutionField(SolutionField&&) = default;
+    SolutionField& operator=(const SolutionField&) = delete;
+    SolutionField& operator=(SolutionField&&) = default;
+    SolutionField(const par::DualShape<2>& s)
+        : interior(s), boundary{VarField(par::DualShape2{{s[1], 2}}), VarField(par::DualShape2{{s[1], 2}}),
+                                VarField(par::DualShape2{{s[0], 2}}), VarField(par::DualShape2{{s[0], 2}})}
     {
-        boundary[0].resize(2 * nj);
-        boundary[1].resize(2 * nj);
-        boundary[2].resize(2 * ni);
-        boundary[3].resize(2 * ni);
     }
-    VarSet& operator()(std::size_t i, std::size_t j) { return interior[i * nj + j]; }
-    const VarSet& operator()(std::size_t i, std::size_t j) const { return interior[i * nj + j]; }
 };
+
+// deep copies (solutionField.h:94-140)
+template <par::execution_policy Policy, typename VarSet, int nDim>
+void copy(const Policy policy, SolutionField<VarSet, nDim>& dst, const SolutionField<VarSet, nDim>& src)
+{
+    par::copy(policy, dst.interior, src.interior);
+    for (unsigned int b = 0; b < src.nBoundaries; ++b) {
+        par::copy(policy, dst.boundary[b], src.boundary[b]);
+        dst.bcTypes[b] = src.bcTypes[b];
+    }
+}
+template <par::execution_policy Policy, typename VarSet, int nDim>
+SolutionField<VarSet, nDim> copy(const Policy policy, const SolutionField<VarSet, nDim>& src)
+{
+    SolutionField<VarSet, nDim> dst(src.interior.shape());
+    copy(policy, dst, src);
+    return dst;
+}
+template <typename VarSet, int nDim> void copy(SolutionField<VarSet, nDim>& dst, const SolutionField<VarSet, nDim>& src)
+{
+    copy(par::execution::seq, dst, src);
+}
+template <typename VarSet, int nDim> SolutionField<VarSet, nDim> copy(const SolutionField<VarSet, nDim>& src)
+{
+    return copy(par::execution::seq, src);
+}
 
 // ---------------- limiters (limiters/limiter.h:14-131) ----------------
 
@@ -400,6 +526,40 @@ State<Law, nDim, Real> set2State(const Species<Law, Real>& species, const Variab
         st.state = {s.q, s.u, s.v};
     }
     return st;
+}
+
+
+// state2Set / set2Set (euler/transforms/{conserved,primitive}.ipp, base/base.h:250-273): the
+// conserved and primitive sets of a State, and a set in one basis from a set in another
+template <typename DstT, LawType Law, int nDim, floating_point Real>
+DstT state2Set(const Species<Law, Real>&, const State<Law, nDim, Real>& st)
+{
+    DstT q;
+    if constexpr (Law == LawType::Euler) {
+        hc::EState<Real> s;
+        s.p = st.pressure();
+        s.r = st.density();
+        s.t = st.temperature();
+        s.h = st.specificTotalEnthalpy();
+        s.v2 = st.velocity2();
+        s.a2 = st.speedOfSound2();
+        s.u = st.velocity(0);
+        s.v = st.velocity(1);
+        if constexpr (hcamd_detail::basis_id<varset_traits<DstT>::basis>() == 1) hc::euler_to_primitive(s, q.elems.data());
+        else hc::euler_to_conserved(s, q.elems.data());
+    } else {
+        q[0] = st.velocity(0);
+        q[1] = st.velocity(1);
+        q[2] = st.scalar();
+    }
+    return q;
+}
+
+template <typename DstT, LawType Law, int nDim, auto Basis, floating_point Real>
+DstT set2Set(const Species<Law, Real>& species, const VariableSet<Law, nDim, Basis, Real>& src)
+{
+    if constexpr (std::is_same_v<DstT, VariableSet<Law, nDim, Basis, Real>>) return src;
+    else return state2Set<DstT>(species, set2State(species, src));
 }
 
 // FluxInterface CRTP: any solution basis -> FluxResult in the conserved basis (base/base.h:370-397)
@@ -552,24 +712,167 @@ template <floating_point Real> struct SteadyTimeControls {
     Real smoothing_factor;
 };
 
-// execution policies (parallalg/parallalg.h:65-87); the work runs on the GPU whichever is passed
-namespace par {
-namespace execution {
-struct sequenced_policy {};
-struct openmp_policy {};
+
+// ---------------- least-squares metrics (lsq/lsq.h:10-90, xmetric.ipp:53-69, qmetric.ipp:48-70) ----------------
+
+namespace lsq {
+// X = sum dx dx^T, upper triangle {xx, xy, yy}
+template <int nDim, floating_point Real> struct XMetric {
+    std::array<Real, nDim * (nDim + 1) / 2> x{};
+    const Real& operator()(const int i) const { return x[i]; }
+    Real& operator()(const int i) { return x[i]; }
+    XMetric& operator+=(const XMetric& o)
+    {
+        for (std::size_t k = 0; k < x.size(); ++k) x[k] = x[k] + o.x[k];
+        return *this;
+    }
+};
+template <int nDim, floating_point Real> XMetric<nDim, Real> operator+(XMetric<nDim, Real> a, const XMetric<nDim, Real>& b)
+{
+    return a += b;
+}
+template <int nDim, floating_point Real> XMetric<nDim, Real> xmetric(const geom::Direction<nDim, Real>& dx)
+{
+    XMetric<nDim, Real> xm;
+    int k = 0;
+    for (int i = 0; i < nDim; ++i)
+        for (int j = i; j < nDim; ++j) xm(k++) = dx[i] * dx[j];
+    return xm;
+}
+
+// Q = sum dq (x) dx, q(var, dim)
+template <typename VarSet> struct QMetric {
+    static constexpr LawType Law = law_of_v<VarSet>;
+    static constexpr int nDim = dim_of_v<VarSet>;
+    static constexpr int nVar = ::nVar<Law, nDim>;
+    using Real = fptype_of_t<VarSet>;
+    std::array<Real, nVar * nDim> q{};
+    const Real& operator()(const int i, const int j) const { return q[i * nDim + j]; }
+    Real& operator()(const int i, const int j) { return q[i * nDim + j]; }
+    QMetric& operator+=(const QMetric& o)
+    {
+        for (std::size_t k = 0; k < q.size(); ++k) q[k] = q[k] + o.q[k];
+        return *this;
+    }
+};
+template <typename VarSet> QMetric<VarSet> operator+(QMetric<VarSet> a, const QMetric<VarSet>& b) { return a += b; }
+template <typename VarSet, int nDim, floating_point Real>
+QMetric<VarSet> qmetric(const geom::Direction<nDim, Real>& dx, const vardelta_t<VarSet>& dq)
+{
+    QMetric<VarSet> qm;
+    for (int i = 0; i < QMetric<VarSet>::nVar; ++i)
+        for (int j = 0; j < nDim; ++j) qm(i, j) = dq[i] * dx[j];
+    return qm;
+}
+}   // namespace lsq
+
+namespace hcamd_detail {
+template <floating_point Real> geom::Direction<2, Real> centre_diff(const geom::Volume<2, Real>& c0, const geom::Volume<2, Real>& c1)
+{
+    return geom::Direction<2, Real>{{c1.centre[0] - c0.centre[0], c1.centre[1] - c0.centre[1]}};
+}
+}   // namespace hcamd_detail
+
+/*
+ * xmetrics(policy, cells[, dxdx]) (spatial/lsqMetrics.h:20-62): X of every cell from its face
+ * neighbours (each interior i-edge, then each interior j-edge, added to both of its cells) and a
+ * mirror point across every side on the boundary (boundary_xmetrics, :104-163).  The reference's
+ * definition on the cell array; the device's metric, held with the fused residual, takes the
+ * periodic image instead of the mirror point across periodic sides (equal without them).
+ */
+template <par::execution_policy Policy, floating_point Real>
+void xmetrics(const Policy policy, const MeshCellArray<2, Real>& cells, par::DualArray<lsq::XMetric<2, Real>, 2>& dxdx)
+{
+    using XM = lsq::XMetric<2, Real>;
+    assert(cells.shape() == dxdx.shape());
+    par::fill(policy, dxdx, XM{});
+    const std::size_t ni = cells.shape(0), nj = cells.shape(1);
+    for (std::size_t j = 0; j < nj; ++j)
+        for (std::size_t i = 0; i + 1 < ni; ++i) {
+            const XM x = lsq::xmetric(hcamd_detail::centre_diff(cells({i, j}), cells({i + 1, j})));
+            dxdx({i, j}) += x;
+            dxdx({i + 1, j}) += x;
+        }
+    for (std::size_t i = 0; i < ni; ++i)
+        for (std::size_t j = 0; j + 1 < nj; ++j) {
+            const XM x = lsq::xmetric(hcamd_detail::centre_diff(cells({i, j}), cells({i, j + 1})));
+            dxdx({i, j}) += x;
+            dxdx({i, j + 1}) += x;
+        }
+    auto mirror = [&](std::size_t i, std::size_t j, std::size_t i1, std::size_t j1) {
+        const auto& c0 = cells({i, j});
+        const auto& c1 = cells({i1, j1});
+        const Real bx = c0.centre[0] + (c0.centre[0] - c1.centre[0]);
+        const Real by = c0.centre[1] + (c0.centre[1] - c1.centre[1]);
+        dxdx({i, j}) += lsq::xmetric(geom::Direction<2, Real>{{bx - c0.centre[0], by - c0.centre[1]}});
+    };
+    for (std::size_t j = 0; j < nj; ++j) {
+        mirror(0, j, 1, j);
+        mirror(ni - 1, j, ni - 2, j);
+    }
+    for (std::size_t i = 0; i < ni; ++i) {
+        mirror(i, 0, i, 1);
+        mirror(i, nj - 1, i, nj - 2);
+    }
+}
+template <par::execution_policy Policy, floating_point Real>
+par::DualArray<lsq::XMetric<2, Real>, 2> xmetrics(const Policy policy, const MeshCellArray<2, Real>& cells)
+{
+    par::DualArray<lsq::XMetric<2, Real>, 2> dxdx(cells.shape());
+    xmetrics(policy, cells, dxdx);
+    return dxdx;
+}
+
+/*
+ * qmetrics(policy, cells, q, dqdx) (spatial/lsqMetrics.h:181-213): Q of every cell from the
+ * differences across its interior edges (i-edges, then j-edges, added to both cells).  The
+ * reference's definition on the arrays; the fused residual adds the periodic images across
+ * periodic sides.
+ */
+template <par::execution_policy Policy, typename SolVarSet, floating_point Real>
+void qmetrics(const Policy policy, const MeshCellArray<2, Real>& cells, const par::DualArray<SolVarSet, 2>& q0,
+              par::DualArray<lsq::QMetric<SolVarSet>, 2>& dqdx)
+{
+    using QM = lsq::QMetric<SolVarSet>;
+    assert(cells.shape() == q0.shape() && cells.shape() == dqdx.shape());
+    par::fill(policy, dqdx, QM{});
+    const std::size_t ni = cells.shape(0), nj = cells.shape(1);
+    auto edge = [&](par::DualIdx2 a, par::DualIdx2 b) {
+        const QM m = lsq::qmetric<SolVarSet>(hcamd_detail::centre_diff(cells(a), cells(b)), q0(b) - q0(a));
+        dqdx(a) += m;
+        dqdx(b) += m;
+    };
+    for (std::size_t j = 0; j < nj; ++j)
+        for (std::size_t i = 0; i + 1 < ni; ++i) edge({i, j}, {i + 1, j});
+    for (std::size_t i = 0; i < ni; ++i)
+        for (std::size_t j = 0; j + 1 < nj; ++j) edge({i, j}, {i, j + 1});
+}
+template <par::execution_policy Policy, typename SolVarSet, floating_point Real>
+par::DualArray<lsq::QMetric<SolVarSet>, 2> qmetrics(const Policy policy, const MeshCellArray<2, Real>& cells,
+                                                   const par::DualArray<SolVarSet, 2>& q0)
+{
+    par::DualArray<lsq::QMetric<SolVarSet>, 2> dqdx(cells.shape());
+    qmetrics(policy, cells, q0, dqdx);
+    return dqdx;
+}
+
+
+// execution policies: the reference's serial and OpenMP policies (par.hpp) select the GPU path
+// with its defaults; hip_policy sets its options explicitly
+namespace par::execution {
 struct hip_policy {
     int device = 0;
     int precision = HC_PRECISION_FP64;   // or HC_PRECISION_MIXED
     int dt_mode = HC_DT_GLOBAL;          // HC_DT_LOCAL: local time stepping with the stage-0 spectral radius
     int graph_steps = 50;
 };
-inline constexpr sequenced_policy seq{};
-inline constexpr openmp_policy omp{};
 inline constexpr hip_policy hip{};
-}   // namespace execution
+}   // namespace par::execution
+namespace par {
+template <> struct is_execution_policy<execution::hip_policy> : std::true_type {};
 }   // namespace par
 
-// ---------------- GPU forwarding ----------------
+// ---------------- device sessions ----------------
 
 namespace hcamd_detail {
 inline void check(int rc, hc_context* ctx, const char* what)
@@ -583,15 +886,12 @@ template <typename Policy> constexpr par::execution::hip_policy hip_of(const Pol
     else return par::execution::hip_policy{};
 }
 
-template <typename BC> constexpr int bc_id(const BC& bc)
-{
-    return static_cast<int>(bc.type());   // enumerators share the numbering of HC_BC_* for Periodic..Fixed
-}
+template <typename BC> constexpr int bc_id(const BC& bc) { return static_cast<int>(bc.type()); }
 
-template <LawType Law, typename... BCs> int lookup_bc(int type, const std::tuple<BCs...>& bcs)
+template <typename... BCs> bool has_bc(int type, const std::tuple<BCs...>& bcs)
 {
-    int found = -1;
-    std::apply([&](const auto&... bc) { ((found = (bc_id(bc) == type ? type : found)), ...); }, bcs);
+    bool found = false;
+    std::apply([&](const auto&... bc) { ((found = found || bc_id(bc) == type), ...); }, bcs);
     return found;
 }
 
@@ -605,37 +905,112 @@ inline int euler_bc_code(int t)   // EulerBCs -> HC_BC_*
     default: return HC_BC_WALL;
     }
 }
-inline int scalar_bc_code(int t)   // ScalarAdvectionBCs -> HC_BC_*
+inline int scalar_bc_code(int t) { return t == 0 ? HC_BC_PERIODIC : t == 1 ? HC_BC_RIEMANN : HC_BC_FIXED; }
+
+// a device context set up for one mesh and one scheme; its key says what it was built from
+struct Session {
+    hc_context* ctx = nullptr;
+    std::uint64_t nodes_hash = 0;
+    std::size_t ni = 0, nj = 0;
+    hc_case_desc desc{};
+    int device = 0, precision = -1, graph_steps = -1;
+    ~Session()
+    {
+        if (ctx) hc_destroy(ctx);
+    }
+};
+
+inline std::uint64_t hash_bytes(const void* p, std::size_t n, std::uint64_t h = 1469598103934665603ULL)
 {
-    return t == 0 ? HC_BC_PERIODIC : t == 1 ? HC_BC_RIEMANN : HC_BC_FIXED;
+    const unsigned char* b = static_cast<const unsigned char*>(p);
+    for (std::size_t k = 0; k < n; ++k) h = (h ^ b[k]) * 1099511628211ULL;
+    return h;
 }
 
-// One GPU context set up from the reference-style arguments.
-template <LawType Law, int nDim, auto Basis, floating_point Real, typename HoFlux, typename... BCs>
-hc_context* setup(const par::execution::hip_policy& pol, const HoFlux& hoflux, const std::tuple<BCs...>& bcs,
-                  const Species<Law, Real>& species, const Mesh<2, Real>& mesh,
-                  const SolutionField<VariableSet<Law, 2, Basis, Real>, 2>& q, int nstages, const double (*alpha)[6],
-                  const double* beta, double cfl)
+// The sessions of this thread, most recent last; a few are kept so that a time loop alternating
+// boundaryUpdate and residualCalc (two schemes on one mesh) rebuilds nothing.
+inline std::vector<std::unique_ptr<Session>>& sessions()
 {
-    static_assert(std::is_same_v<Real, double>, "the GPU path takes FP64 state");
-    hc_context* ctx = nullptr;
-    if (hc_create(&ctx, pol.device) != HC_OK) throw std::runtime_error("hc_create failed");
-    std::vector<double> nodes(2 * mesh.nodes.size());
-    for (std::size_t k = 0; k < mesh.nodes.size(); ++k) {
-        nodes[2 * k] = mesh.nodes[k][0];
-        nodes[2 * k + 1] = mesh.nodes[k][1];
+    thread_local std::vector<std::unique_ptr<Session>> s;
+    return s;
+}
+constexpr std::size_t MAX_SESSIONS = 4;
+
+// The session for (mesh, case descriptor, policy): reused when the mesh nodes, the descriptor and
+// the device match, else created (geometry built once on hc_set_case).
+inline hc_context* session(const par::execution::hip_policy& pol, const double* nodes, std::size_t ni, std::size_t nj,
+                           const hc_case_desc& d)
+{
+    const std::uint64_t h = hash_bytes(nodes, 2 * (ni + 1) * (nj + 1) * sizeof(double));
+    auto& all = sessions();
+    for (std::size_t k = 0; k < all.size(); ++k) {
+        Session& s = *all[k];
+        if (s.nodes_hash == h && s.ni == ni && s.nj == nj && s.device == pol.device &&
+            std::memcmp(&s.desc, &d, sizeof(d)) == 0) {
+            std::unique_ptr<Session> keep = std::move(all[k]);
+            all.erase(all.begin() + k);
+            all.push_back(std::move(keep));
+            Session& t = *all.back();
+            if (t.precision != pol.precision) {
+                check(hc_set_precision(t.ctx, pol.precision), t.ctx, "hc_set_precision");
+                t.precision = pol.precision;
+            }
+            if (t.graph_steps != pol.graph_steps) {
+                check(hc_set_graph_steps(t.ctx, pol.graph_steps), t.ctx, "hc_set_graph_steps");
+                t.graph_steps = pol.graph_steps;
+            }
+            return t.ctx;
+        }
     }
-    check(hc_set_mesh(ctx, (int)mesh.ni, (int)mesh.nj, nodes.data()), ctx, "hc_set_mesh");
-    hc_case_desc d{};
+    if (all.size() >= MAX_SESSIONS) all.erase(all.begin());
+    auto s = std::make_unique<Session>();
+    if (hc_create(&s->ctx, pol.device) != HC_OK) throw std::runtime_error("hc_create failed");
+    check(hc_set_mesh(s->ctx, (int)ni, (int)nj, nodes), s->ctx, "hc_set_mesh");
+    check(hc_set_case(s->ctx, &d), s->ctx, "hc_set_case");
+    check(hc_set_precision(s->ctx, pol.precision), s->ctx, "hc_set_precision");
+    check(hc_set_graph_steps(s->ctx, pol.graph_steps), s->ctx, "hc_set_graph_steps");
+    s->nodes_hash = h;
+    s->ni = ni;
+    s->nj = nj;
+    s->desc = d;
+    s->device = pol.device;
+    s->precision = pol.precision;
+    s->graph_steps = pol.graph_steps;
+    all.push_back(std::move(s));
+    return all.back()->ctx;
+}
+
+template <floating_point Real> std::vector<double> flat_nodes(const Mesh<2, Real>& mesh)
+{
+    static_assert(std::is_same_v<Real, double>, "the GPU path takes FP64 geometry");
+    const auto& n = mesh.nodes.flatten();
+    std::vector<double> out(2 * n.size());
+    for (std::size_t k = 0; k < n.size(); ++k) {
+        out[2 * k] = n[k][0];
+        out[2 * k + 1] = n[k][1];
+    }
+    return out;
+}
+
+// the case descriptor of a scheme: flux / limiter / scalings (make_muscl_flux), the boundary types
+// of the field and the tuple that must hold them, the Runge-Kutta table and CFL, the species
+template <LawType Law, auto Basis, floating_point Real, typename... BCs>
+hc_case_desc describe(int flux_id, int vscale, int pscale, int limiter_id, const std::tuple<BCs...>& bcs,
+                      const std::array<BoundaryType<Law>, 4>& bcTypes, const Species<Law, Real>& species,
+                      const Mesh<2, Real>& mesh, int nstages, const double (*alpha)[6], const double* beta, double cfl,
+                      int dt_mode)
+{
+    hc_case_desc d;
+    std::memset(&d, 0, sizeof(d));
     d.law = Law == LawType::Euler ? HC_LAW_EULER : HC_LAW_SCALAR;
     d.basis = basis_id<Basis>();
-    d.flux = HoFlux::flux_id;
-    d.vscale = HoFlux::vscale;
-    d.pscale = HoFlux::pscale;
-    d.limiter = HoFlux::limiter_id;
+    d.flux = flux_id;
+    d.vscale = vscale;
+    d.pscale = pscale;
+    d.limiter = limiter_id;
     for (int b = 0; b < 4; ++b) {
-        const int t = q.bcTypes[b];
-        if (lookup_bc<Law>(t, bcs) < 0) throw std::runtime_error("no boundary condition in the tuple matches bcTypes");
+        const int t = static_cast<int>(bcTypes[b]);
+        if (!has_bc(t, bcs)) throw std::runtime_error("no boundary condition in the tuple matches bcTypes");
         d.bc[b] = Law == LawType::Euler ? euler_bc_code(t) : scalar_bc_code(t);
     }
     d.nstages = nstages;
@@ -644,7 +1019,7 @@ hc_context* setup(const par::execution::hip_policy& pol, const HoFlux& hoflux, c
         for (int k = 0; k < 6; ++k) d.alpha[s][k] = alpha[s][k];
     }
     d.cfl = cfl;
-    d.dt_mode = pol.dt_mode;
+    d.dt_mode = dt_mode;
     d.period_i[0] = mesh.period_i[0];
     d.period_i[1] = mesh.period_i[1];
     d.period_j[0] = mesh.period_j[0];
@@ -652,111 +1027,281 @@ hc_context* setup(const par::execution::hip_policy& pol, const HoFlux& hoflux, c
     if constexpr (Law == LawType::Euler)
         d.sp = hc_species{species.gamma, species.minf, species.lref, species.nu, species.pr, species.dt, species.R,
                           species.gamma1};
-    check(hc_set_case(ctx, &d), ctx, "hc_set_case");
-    check(hc_set_precision(ctx, pol.precision), ctx, "hc_set_precision");
-    constexpr int nv = nVar<Law, 2>;
+    return d;
+}
+
+// boundary reference values (k, 1) of the field in the C-ABI's row order
+template <typename VarSet> std::vector<double> boundary_refs(const SolutionField<VarSet, 2>& q)
+{
+    constexpr int nv = VarSet::N;
     std::vector<double> bref;
-    for (int b = 0; b < 4; ++b) {
-        const std::size_t n_along = b < 2 ? q.nj : q.ni;
-        for (std::size_t k = 0; k < n_along; ++k)
-            for (int v = 0; v < nv; ++v) bref.push_back(q.boundary[b][2 * k + 1][v]);
+    for (int b = 0; b < 4; ++b)
+        for (std::size_t k = 0; k < q.boundary[b].shape(0); ++k)
+            for (int v = 0; v < nv; ++v) bref.push_back(q.boundary[b]({k, 1})[v]);
+    return bref;
+}
+
+template <typename VarSet> const double* raw(const par::DualArray<VarSet, 2>& a)
+{
+    static_assert(sizeof(VarSet) == sizeof(double) * VarSet::N, "packed variable sets");
+    return reinterpret_cast<const double*>(a.data());
+}
+template <typename VarSet> double* raw(par::DualArray<VarSet, 2>& a)
+{
+    static_assert(sizeof(VarSet) == sizeof(double) * VarSet::N, "packed variable sets");
+    return reinterpret_cast<double*>(a.data());
+}
+
+template <floating_point Real> void rk_arrays(const ODE::Explicit::RungeKutta<Real>& rk, double (*alpha)[6], double* beta)
+{
+    for (int s = 0; s < 6; ++s) {
+        beta[s] = rk.beta[s];
+        for (int k = 0; k < 6; ++k) alpha[s][k] = rk.alpha[s][k];
     }
+}
+}   // namespace hcamd_detail
+
+// ---------------- spatial entry points ----------------
+
+/*
+ * boundaryUpdate(mesh, bcs, species, q) (spatial/boundary/boundaryUpdate.h:28-48, 137-266): ghost
+ * values q.boundary[b](k, 0) of every non-periodic boundary from the interior and the reference
+ * values (Riemann / entropy far field, Fixed, inviscid wall), computed on the GPU.
+ */
+template <LawType Law, int nDim, floating_point Real, typename SolVarT, typename... BoundaryConds>
+void boundaryUpdate(const Mesh<nDim, Real>& mesh, const std::tuple<BoundaryConds...> bcs, const Species<Law, Real>& species,
+                    SolutionField<SolVarT, nDim>& q)
+{
+    static_assert(nDim == 2, "two-dimensional meshes");
+    assert(mesh.cells.shape() == q.interior.shape());
+    const auto rk = ODE::Explicit::ssp11<Real>();
+    double alpha[6][6], beta[6];
+    hcamd_detail::rk_arrays(rk, alpha, beta);
+    const hc_case_desc d = hcamd_detail::describe<Law, varset_traits<SolVarT>::basis>(
+        HC_FLUX_CENTRAL, 0, 0, HC_LIM_NOLIMIT1, bcs, q.bcTypes, species, mesh, 1, alpha, beta, 1.0, HC_DT_GLOBAL);
+    const auto nodes = hcamd_detail::flat_nodes(mesh);
+    hc_context* ctx = hcamd_detail::session(par::execution::hip_policy{}, nodes.data(), mesh.cells.shape(0),
+                                            mesh.cells.shape(1), d);
+    constexpr int nv = SolVarT::N;
+    const std::vector<double> bref = hcamd_detail::boundary_refs(q);
+    std::vector<double> ghost(bref.size());
+    std::size_t row = 0;
+    for (int b = 0; b < 4; ++b)
+        for (std::size_t k = 0; k < q.boundary[b].shape(0); ++k, ++row)
+            for (int v = 0; v < nv; ++v) ghost[row * nv + v] = q.boundary[b]({k, 0})[v];
+    hcamd_detail::check(hc_boundary_update(ctx, hcamd_detail::raw(q.interior), bref.data(), ghost.data()), ctx,
+                        "hc_boundary_update");
+    row = 0;
+    for (int b = 0; b < 4; ++b)
+        for (std::size_t k = 0; k < q.boundary[b].shape(0); ++k, ++row)
+            for (int v = 0; v < nv; ++v) q.boundary[b]({k, 0})[v] = ghost[row * nv + v];
+}
+
+/*
+ * residualCalc(policy, hoflux, bcs, species, mesh, q, dxdx, dqdx, res) (spatial/residualCalc.h:27-69):
+ * R_i = -sum_faces F A and lambda_i = sum_faces lambda_f A over the interior and boundary faces in
+ * the reference's order, on the GPU.  The fused kernel forms its LSQ metrics itself (the X metric
+ * once with the geometry, the Q metric per call from q, with the periodic images across periodic
+ * sides); dxdx and dqdx must have the mesh's shape.  The ghost values are computed from the
+ * interior and reference values as boundaryUpdate does.
+ */
+template <typename Policy, LawType Law, int nDim, typename SolVarT, typename FluxRes, typename HighOrderFlux,
+          typename... BoundaryConds, floating_point Real>
+void residualCalc(const Policy policy, const HighOrderFlux& hoflux, const std::tuple<BoundaryConds...> bcs,
+                  const Species<Law, Real>& species, const Mesh<nDim, Real>& mesh, const SolutionField<SolVarT, nDim>& q,
+                  const par::DualArray<lsq::XMetric<nDim, Real>, nDim>& dxdx,
+                  const par::DualArray<lsq::QMetric<SolVarT>, nDim>& dqdx, par::DualArray<FluxRes, nDim>& res)
+{
+    static_assert(std::is_same_v<FluxRes, FluxResult<Law, nDim, Real>>, "residual array of FluxResult");
+    assert(mesh.cells.shape() == q.interior.shape() && mesh.cells.shape() == dxdx.shape() &&
+           mesh.cells.shape() == dqdx.shape() && mesh.cells.shape() == res.shape());
+    (void)hoflux;
+    const auto rk = ODE::Explicit::ssp11<Real>();
+    double alpha[6][6], beta[6];
+    hcamd_detail::rk_arrays(rk, alpha, beta);
+    const hc_case_desc d = hcamd_detail::describe<Law, varset_traits<SolVarT>::basis>(
+        HighOrderFlux::flux_id, HighOrderFlux::vscale, HighOrderFlux::pscale, HighOrderFlux::limiter_id, bcs, q.bcTypes,
+        species, mesh, 1, alpha, beta, 1.0, HC_DT_GLOBAL);
+    const auto nodes = hcamd_detail::flat_nodes(mesh);
+    const auto pol = hcamd_detail::hip_of(policy);
+    hc_context* ctx = hcamd_detail::session(pol, nodes.data(), mesh.cells.shape(0), mesh.cells.shape(1), d);
+    const std::vector<double> bref = hcamd_detail::boundary_refs(q);
+    hcamd_detail::check(hc_set_boundary_refs(ctx, bref.data()), ctx, "hc_set_boundary_refs");
+    hcamd_detail::check(hc_set_state(ctx, hcamd_detail::raw(q.interior)), ctx, "hc_set_state");
+    static_assert(sizeof(FluxRes) == sizeof(double) * (SolVarT::N + 1), "packed flux results");
+    hcamd_detail::check(hc_residual(ctx, reinterpret_cast<double*>(res.data())), ctx, "hc_residual");
+}
+
+/*
+ * spectralRadius(policy, cells, res) (spatial/spectralRadius.h:19-48): max over cells of
+ * lambda / volume, initial value numeric_limits<Real>::min() (:42).
+ */
+template <typename Policy, int nDim, typename FluxRes, floating_point Real>
+Real spectralRadius(const Policy policy, const MeshCellArray<nDim, Real>& cells, const par::DualArray<FluxRes, nDim>& res)
+{
+    (void)policy;
+    return par::transform_reduce(
+        par::execution::seq, [](const geom::Volume<nDim, Real>& c, const FluxRes& r) { return r.lambda / c.volume; },
+        [](const Real l, const Real r) { return std::max(l, r); }, std::numeric_limits<Real>::min(), cells, res);
+}
+
+/*
+ * rungeKuttaAccumulation(policy, rk, stg, resStage, resTotal) (timestepping/rungeKutta.h:176-197):
+ * resTotal.flux = sum_{k <= stg} alpha[stg][k] resStage[k].flux in that order; lambda stays 0.
+ */
+template <typename Policy, int nDim, typename FluxRes, floating_point Real>
+void rungeKuttaAccumulation(const Policy policy, const ODE::Explicit::RungeKutta<Real>& rk, const std::size_t stg,
+                            const std::vector<par::DualArray<FluxRes, nDim>>& resStage,
+                            par::DualArray<FluxRes, nDim>& resTotal)
+{
+    (void)policy;
+    par::fill(resTotal, FluxRes{});
+    const std::size_t n = resTotal.flattened_length();
+    for (std::size_t c = 0; c < n; ++c) {
+        FluxRes& t = resTotal.flatten(c);
+        for (std::size_t k = 0; k <= stg; ++k) t.flux = t.flux + rk.alpha[stg][k] * resStage[k].flatten(c).flux;
+    }
+}
+
+namespace hcamd_detail {
+template <LawType Law, auto Basis, floating_point Real, typename SolVarT>
+SolVarT forward(const Species<Law, Real>& species, const SolVarT& v0, const Real* R, const Real fac)
+{
+    constexpr int L = Law == LawType::Euler ? hc::LAW_EULER : hc::LAW_SCALAR;
+    constexpr int B = basis_id<Basis>();
+    const auto sp = hcamd_detail::species(species);
+    Real qc[SolVarT::N];
+    hc::to_conserved<Real, L, B>(sp, v0.elems.data(), qc);
+    for (int v = 0; v < SolVarT::N; ++v) qc[v] = qc[v] + R[v] * fac;
+    SolVarT v1;
+    hc::from_conserved<Real, L, B>(sp, qc, v1.elems.data());
+    return v1;
+}
+}   // namespace hcamd_detail
+
+/*
+ * eulerForwardUpdateGlobal(policy, cells, species, dt, r, q0, q1) (spatial/eulerForwardUpdate.h:38-92):
+ * q1 = S(C(q0) + r.flux (dt / volume)), S and C the solution-basis <-> conserved transforms.
+ */
+template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT>
+void eulerForwardUpdateGlobal(const Policy policy, const MeshCellArray<nDim, Real>& cells, const Species<Law, Real>& species,
+                              const Real dt, const par::DualArray<FluxResult<Law, nDim, Real>, nDim>& r,
+                              const par::DualArray<SolVarT, nDim>& q0, par::DualArray<SolVarT, nDim>& q1)
+{
+    (void)policy;
+    par::transform(
+        [&](const SolVarT& v0, const geom::Volume<nDim, Real>& c, const FluxResult<Law, nDim, Real>& res) {
+            return hcamd_detail::forward<Law, varset_traits<SolVarT>::basis>(species, v0, res.flux.elems.data(),
+                                                                            dt / c.volume);
+        },
+        q1, q0, cells, r);
+}
+
+/*
+ * eulerForwardUpdateLocal(policy, species, cfl, r, q0, q1) (spatial/eulerForwardUpdate.h:117-168):
+ * q1 = S(C(q0) + r.flux (cfl / r.lambda)); the lambda of an accumulated residual is 0 in the
+ * reference (rungeKutta.h:176-197), so callers pass the stage-0 spectral radius there.
+ */
+template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT>
+void eulerForwardUpdateLocal(const Policy policy, const Species<Law, Real>& species, const Real cfl,
+                             const par::DualArray<FluxResult<Law, nDim, Real>, nDim>& r,
+                             const par::DualArray<SolVarT, nDim>& q0, par::DualArray<SolVarT, nDim>& q1)
+{
+    (void)policy;
+    par::transform(
+        [&](const SolVarT& v0, const FluxResult<Law, nDim, Real>& res) {
+            return hcamd_detail::forward<Law, varset_traits<SolVarT>::basis>(species, v0, res.flux.elems.data(),
+                                                                            cfl / res.lambda);
+        },
+        q1, q0, r);
+}
+
+// ---------------- time integration on the GPU ----------------
+
+namespace hcamd_detail {
+template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT, typename SecondOrderFlux,
+          typename... BoundaryConds>
+hc_context* integrate_session(const Policy policy, const ODE::Explicit::RungeKutta<Real>& rk, Real cfl,
+                              const SecondOrderFlux& flux2, const std::tuple<BoundaryConds...>& bcs,
+                              const Species<Law, Real>& species, const Mesh<nDim, Real>& mesh,
+                              const SolutionField<SolVarT, nDim>& q0)
+{
+    static_assert(nDim == 2, "two-dimensional meshes");
+    assert(q0.interior.shape() == mesh.cells.shape());
+    (void)flux2;
+    double alpha[6][6], beta[6];
+    rk_arrays(rk, alpha, beta);
+    const auto pol = hip_of(policy);
+    const hc_case_desc d = describe<Law, varset_traits<SolVarT>::basis>(
+        SecondOrderFlux::flux_id, SecondOrderFlux::vscale, SecondOrderFlux::pscale, SecondOrderFlux::limiter_id, bcs,
+        q0.bcTypes, species, mesh, (int)rk.nstages, alpha, beta, cfl, pol.dt_mode);
+    const auto nodes = flat_nodes(mesh);
+    hc_context* ctx = session(pol, nodes.data(), mesh.cells.shape(0), mesh.cells.shape(1), d);
+    const std::vector<double> bref = boundary_refs(q0);
     check(hc_set_boundary_refs(ctx, bref.data()), ctx, "hc_set_boundary_refs");
-    check(hc_set_state(ctx, reinterpret_cast<const double*>(q.interior.data())), ctx, "hc_set_state");
-    check(hc_set_graph_steps(ctx, pol.graph_steps), ctx, "hc_set_graph_steps");
+    check(hc_set_state(ctx, raw(q0.interior)), ctx, "hc_set_state");
+    unsigned int status = 0;
+    check(hc_get_status(ctx, &status), ctx, "hc_get_status");   // clear what earlier calls left
     return ctx;
 }
 }   // namespace hcamd_detail
 
 /*
  * integrate(policy, timeControls, rungeKutta, flux2, boundaryConds, species, mesh, q0)
- * (timestepping/rungeKutta.h:37-167): runs the time loop on the GPU and writes the final
- * interior state back into q0.  Returns the accumulated time.
+ * (timestepping/rungeKutta.h:44-167): the whole time loop on the GPU (hc_step: fused stage
+ * kernels replayed from HIP graphs); the final interior state is written back into q0.  Returns
+ * the accumulated time of a global time step (0 for a local one).
  */
-template <typename Policy, LawType Law, int nDim, floating_point Real, auto Basis, typename SecondOrderFlux,
+template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT, typename SecondOrderFlux,
           typename... BoundaryConds>
+    requires par::execution_policy<Policy>
 Real integrate(const Policy policy, const UnsteadyTimeControls<Real>& timeControls,
                const ODE::Explicit::RungeKutta<Real>& rungeKutta, const SecondOrderFlux& flux2,
                const std::tuple<BoundaryConds...> boundaryConds, const Species<Law, Real>& species,
-               const Mesh<nDim, Real>& mesh, SolutionField<VariableSet<Law, nDim, Basis, Real>, nDim>& q0)
+               const Mesh<nDim, Real>& mesh, SolutionField<SolVarT, nDim>& q0)
 {
-    static_assert(sizeof(VariableSet<Law, nDim, Basis, Real>) == sizeof(Real) * nVar<Law, nDim>, "packed variable sets");
-    double alpha[6][6], beta[6];
-    for (int s = 0; s < 6; ++s) {
-        beta[s] = rungeKutta.beta[s];
-        for (int k = 0; k < 6; ++k) alpha[s][k] = rungeKutta.alpha[s][k];
-    }
-    hc_context* ctx = hcamd_detail::setup<Law, nDim, Basis>(hcamd_detail::hip_of(policy), flux2, boundaryConds, species,
-                                                            mesh, q0, (int)rungeKutta.nstages, alpha, beta,
-                                                            timeControls.cfl);
+    hc_context* ctx = hcamd_detail::integrate_session(policy, rungeKutta, timeControls.cfl, flux2, boundaryConds,
+                                                      species, mesh, q0);
     hcamd_detail::check(hc_step(ctx, (int)timeControls.nTimesteps), ctx, "hc_step");
-    hcamd_detail::check(hc_get_state(ctx, reinterpret_cast<double*>(q0.interior.data())), ctx, "hc_get_state");
+    hcamd_detail::check(hc_get_state(ctx, hcamd_detail::raw(q0.interior)), ctx, "hc_get_state");
     double t = 0.0;
     hcamd_detail::check(hc_get_time(ctx, &t), ctx, "hc_get_time");
     unsigned int status = 0;
-    hc_get_status(ctx, &status);
-    hc_destroy(ctx);
+    hcamd_detail::check(hc_get_status(ctx, &status), ctx, "hc_get_status");
     if (status) throw std::runtime_error("device raised status bits " + std::to_string(status));
     return Real(t);
 }
 
+// the reference drivers' call without a policy (script/euler/cylinder2D.cpp:184-190)
+template <LawType Law, int nDim, floating_point Real, typename SolVarT, typename SecondOrderFlux, typename... BoundaryConds>
+Real integrate(const UnsteadyTimeControls<Real>& timeControls, const ODE::Explicit::RungeKutta<Real>& rungeKutta,
+               const SecondOrderFlux& flux2, const std::tuple<BoundaryConds...> boundaryConds,
+               const Species<Law, Real>& species, const Mesh<nDim, Real>& mesh, SolutionField<SolVarT, nDim>& q0)
+{
+    return integrate(par::execution::hip, timeControls, rungeKutta, flux2, boundaryConds, species, mesh, q0);
+}
+
 /*
- * Steady-state integrate with SteadyTimeControls (controls.h:31-48, for which the reference has no
- * integrate at HEAD): marches until the mass residual has dropped by timeControls.residual_drop
- * (hc_step_steady, checked every 50 steps) or nTimesteps are taken, writes the state back into q0
- * and returns the drop reached.  Residual smoothing is not built.
+ * Steady-state integrate with SteadyTimeControls (controls.h:31-48, no integrate consumes it at the
+ * reference's HEAD): marches until the mass residual's L2 norm has dropped by
+ * timeControls.residual_drop (checked every 50 steps) or nTimesteps are taken, writes the state
+ * back into q0 and returns the drop reached.  Residual smoothing is not built.
  */
-template <typename Policy, LawType Law, int nDim, floating_point Real, auto Basis, typename SecondOrderFlux,
+template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT, typename SecondOrderFlux,
           typename... BoundaryConds>
+    requires par::execution_policy<Policy>
 Real integrate(const Policy policy, const SteadyTimeControls<Real>& timeControls,
                const ODE::Explicit::RungeKutta<Real>& rungeKutta, const SecondOrderFlux& flux2,
                const std::tuple<BoundaryConds...> boundaryConds, const Species<Law, Real>& species,
-               const Mesh<nDim, Real>& mesh, SolutionField<VariableSet<Law, nDim, Basis, Real>, nDim>& q0)
+               const Mesh<nDim, Real>& mesh, SolutionField<SolVarT, nDim>& q0)
 {
     if (timeControls.use_residual_smoothing) throw std::runtime_error("residual smoothing is not built");
-    double alpha[6][6], beta[6];
-    for (int s = 0; s < 6; ++s) {
-        beta[s] = rungeKutta.beta[s];
-        for (int k = 0; k < 6; ++k) alpha[s][k] = rungeKutta.alpha[s][k];
-    }
-    hc_context* ctx = hcamd_detail::setup<Law, nDim, Basis>(hcamd_detail::hip_of(policy), flux2, boundaryConds, species,
-                                                            mesh, q0, (int)rungeKutta.nstages, alpha, beta,
-                                                            timeControls.cfl);
+    hc_context* ctx = hcamd_detail::integrate_session(policy, rungeKutta, timeControls.cfl, flux2, boundaryConds,
+                                                      species, mesh, q0);
     long long steps = 0;
     double drop = 1.0;
     hcamd_detail::check(hc_step_steady(ctx, (long long)timeControls.nTimesteps, timeControls.residual_drop, 50, &steps,
                                        &drop), ctx, "hc_step_steady");
-    hcamd_detail::check(hc_get_state(ctx, reinterpret_cast<double*>(q0.interior.data())), ctx, "hc_get_state");
-    hc_destroy(ctx);
+    hcamd_detail::check(hc_get_state(ctx, hcamd_detail::raw(q0.interior)), ctx, "hc_get_state");
     return Real(drop);
-}
-
-/*
- * residualCalc(policy, hoflux, bcs, species, mesh, q, res) (spatial/residualCalc.h:45-69);
- * the LSQ metric arrays of the reference signature are computed on the device.
- */
-template <typename Policy, LawType Law, int nDim, floating_point Real, auto Basis, typename HighOrderFlux,
-          typename... BoundaryConds>
-void residualCalc(const Policy policy, const HighOrderFlux& hoflux, const std::tuple<BoundaryConds...> bcs,
-                  const Species<Law, Real>& species, const Mesh<nDim, Real>& mesh,
-                  const SolutionField<VariableSet<Law, nDim, Basis, Real>, nDim>& q,
-                  std::vector<FluxResult<Law, nDim, Real>>& res)
-{
-    const auto rk = ODE::Explicit::ssp11<Real>();
-    double alpha[6][6], beta[6];
-    for (int s = 0; s < 6; ++s) {
-        beta[s] = rk.beta[s];
-        for (int k = 0; k < 6; ++k) alpha[s][k] = rk.alpha[s][k];
-    }
-    hc_context* ctx = hcamd_detail::setup<Law, nDim, Basis>(hcamd_detail::hip_of(policy), hoflux, bcs, species, mesh, q,
-                                                            1, alpha, beta, 1.0);
-    constexpr int nv = nVar<Law, nDim>;
-    std::vector<double> r((nv + 1) * q.ni * q.nj);
-    hcamd_detail::check(hc_residual(ctx, r.data()), ctx, "hc_residual");
-    hc_destroy(ctx);
-    res.resize(q.ni * q.nj);
-    for (std::size_t c = 0; c < res.size(); ++c) {
-        for (int v = 0; v < nv; ++v) res[c].flux[v] = r[c * (nv + 1) + v];
-        res[c].lambda = r[c * (nv + 1) + nv];
-    }
 }

@@ -27,6 +27,7 @@ struct CaseHost {
     int ni = 0, nj = 0;
     hc_case_desc cs{};
     GeomOutput g;                  // host geometry until it is uploaded
+    std::vector<double> X;         // LSQ spatial metric per cell {xx, xy, yy}, kept for hc_xmetrics
     long long N = 0, nfi = 0, nfj = 0;
     long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
     int brows = 0;                 // boundary reference rows 2 nj + 2 ni
@@ -92,6 +93,9 @@ struct hc_context {
     std::vector<Pending> ev_pending;
     // tangent / adjoint buffers, created on first use
     AdjState* adj = nullptr;
+    // device staging buffer of the spatial building blocks (hc_qmetrics ... hc_forward_update)
+    double* blk = nullptr;
+    size_t blk_n = 0;
 };
 
 

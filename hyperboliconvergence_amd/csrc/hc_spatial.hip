@@ -1,0 +1,355 @@
+// hc_spatial.hip -- the reference's spatial building blocks as device kernels on a single-case
+// context: Q metric, ghost-value update, spectral radius and the forward-Euler updates.  They
+// serve callers that run the time loop themselves on host arrays (the C++ drop-in's xmetrics /
+// qmetrics / boundaryUpdate / spectralRadius / eulerForwardUpdateGlobal / Local); hc_step runs
+// the same arithmetic fused into the stage kernel.  Host arrays are staged through the context's
+// device buffers between steps (those buffers hold no step state then).  Compiled with
+// -ffp-contract=off: each result is bit-identical to the fused kernel's and to the oracle's.
+#include "hc_context.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <vector>
+
+namespace {
+
+using namespace hc;
+
+// Q metric of every cell (spatial/lsqMetrics.h:181-213; lsq/qmetric.ipp:48-70): per cell the
+// interior i-edges, the interior j-edges, then the periodic wraps, dq = q_R - q_L times the
+// centre difference of the face slots (the stage kernel's qmetric phase, in the same order)
+template <int LAW>
+__global__ void __launch_bounds__(256) k_qmetrics(const MeshView<double> m, const double* __restrict__ q, long long N,
+                                                  double* __restrict__ Q)
+{
+    constexpr int NV = LawTraits<LAW>::nv;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const int ni = m.ni, nj = m.nj;
+    const int i = (int)(c / nj), j = (int)(c % nj);
+    double acc[NV][2];
+    for (int v = 0; v < NV; ++v) acc[v][0] = acc[v][1] = 0.0;
+    auto edge = [&](long long cl, long long cr, const double* base, long long nf, long long fidx) {
+        const double dx0 = base[FS_DX0 * nf + fidx], dx1 = base[FS_DX1 * nf + fidx];
+        for (int v = 0; v < NV; ++v) {
+            const double dq = q[cr * NV + v] - q[cl * NV + v];
+            acc[v][0] = acc[v][0] + dq * dx0;
+            acc[v][1] = acc[v][1] + dq * dx1;
+        }
+    };
+    const long long up = (long long)(i - 1) * nj + j, dn = (long long)(i + 1) * nj + j;
+    if (i > 0) edge(up, c, m.fi, m.nfi, (long long)i * nj + j);
+    if (i < ni - 1) edge(c, dn, m.fi, m.nfi, (long long)(i + 1) * nj + j);
+    if (j > 0) edge(c - 1, c, m.fj, m.nfj, (long long)i * (nj + 1) + j);
+    if (j < nj - 1) edge(c, c + 1, m.fj, m.nfj, (long long)i * (nj + 1) + j + 1);
+    if (m.per_i) {
+        if (i == 0) edge((long long)(ni - 1) * nj + j, c, m.fi, m.nfi, (long long)j);
+        if (i == ni - 1) edge(c, (long long)j, m.fi, m.nfi, (long long)j);
+    }
+    if (m.per_j) {
+        if (j == 0) edge((long long)i * nj + nj - 1, c, m.fj, m.nfj, (long long)i * (nj + 1));
+        if (j == nj - 1) edge(c, (long long)i * nj, m.fj, m.nfj, (long long)i * (nj + 1));
+    }
+    for (int v = 0; v < NV; ++v) {
+        Q[(c * NV + v) * 2 + 0] = acc[v][0];
+        Q[(c * NV + v) * 2 + 1] = acc[v][1];
+    }
+}
+
+// ghost values of the boundary cells (spatial/boundary/boundaryUpdate.h:28-48, 137-266): row r of
+// [2 nj + 2 ni] (boundaries 0, 1 along j, then 2, 3 along i); the face normal points into the
+// domain; Riemann / entropy far field from the point functions, Fixed and the inviscid wall
+// return the reference value (boundaryCondition.h:63-80, inviscidWall.ipp:15-28); periodic rows
+// are not written
+template <int LAW, int BASIS>
+__global__ void __launch_bounds__(256) k_boundary_update(const MeshView<double> m, Species<double> sp, int4 bc,
+                                                         const double* __restrict__ q, const double* __restrict__ bref,
+                                                         double* __restrict__ ghost, unsigned int* status)
+{
+    constexpr int NV = LawTraits<LAW>::nv;
+    const int ni = m.ni, nj = m.nj;
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= 2 * nj + 2 * ni) return;
+    int b, t;
+    if (r < 2 * nj) {
+        b = r < nj ? 0 : 1;
+        t = r - b * nj;
+    } else {
+        b = r - 2 * nj < ni ? 2 : 3;
+        t = r - 2 * nj - (b - 2) * ni;
+    }
+    const int type = b == 0 ? bc.x : b == 1 ? bc.y : b == 2 ? bc.z : bc.w;
+    if (type == BC_PERIODIC) return;
+    double* out = ghost + (long long)r * NV;
+    const double* qref = bref + (long long)r * NV;
+    if (type == BC_WALL) {
+        for (int v = 0; v < NV; ++v) out[v] = qref[v];
+        return;
+    }
+    const bool isi = b < 2;
+    const int l = isi ? (b == 0 ? 0 : ni) : (b == 2 ? 0 : nj);            // node line of the face
+    const long long fidx = isi ? (long long)l * nj + t : (long long)t * (nj + 1) + l;
+    const double* base = isi ? m.fi : m.fj;
+    const long long nf = isi ? m.nfi : m.nfj;
+    const long long cint = isi ? (long long)(b == 0 ? 0 : ni - 1) * nj + t : (long long)t * nj + (b == 2 ? 0 : nj - 1);
+    FaceG<double> f{base[FS_AREA * nf + fidx], base[FS_N0 * nf + fidx], base[FS_N1 * nf + fidx]};
+    if (b == 1 || b == 3) {
+        f.n0 = -f.n0;
+        f.n1 = -f.n1;
+    }
+    double qint[NV];
+    for (int v = 0; v < NV; ++v) qint[v] = q[cint * NV + v];
+    if (!ghost_value<double, LAW, BASIS>(sp, type, f, base[FS_CX * nf + fidx], base[FS_CY * nf + fidx], m.ccx[cint],
+                                         m.ccy[cint], qint, qref, out))
+        atomicOr(status, ST_BCNORMAL);
+}
+
+// max over cells of lambda / volume (spatial/spectralRadius.h:19-48; initial value
+// numeric_limits::min(), :42): per work-group maxima, then the maximum of those (order-free)
+__global__ void __launch_bounds__(256) k_specrad_partials(const double* __restrict__ res, const double* __restrict__ vol,
+                                                          long long N, int nv, double* part)
+{
+    __shared__ double sm[256];
+    double m = DBL_MIN;
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += (long long)gridDim.x * 256)
+        m = fmax(m, res[c * (nv + 1) + nv] / vol[c]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sm[0];
+}
+
+__global__ void __launch_bounds__(256) k_specrad_final(const double* part, int nparts, double* out)
+{
+    __shared__ double sm[256];
+    double m = DBL_MIN;
+    for (int p = threadIdx.x; p < nparts; p += 256) m = fmax(m, part[p]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+// q1 = S( C(q0) + R (coef / vol) ) with mode 0 (eulerForwardUpdateGlobal, eulerForwardUpdate.h:38-92,
+// coef = dt), or q1 = S( C(q0) + R (coef / lambda) ) with mode 1 (eulerForwardUpdateLocal, :117-168,
+// coef = cfl, lambda of the given residual)
+template <int LAW, int BASIS>
+__global__ void __launch_bounds__(256) k_forward_update(int mode, double coef, Species<double> sp,
+                                                        const double* __restrict__ vol, const double* __restrict__ res,
+                                                        const double* __restrict__ q0, long long N,
+                                                        double* __restrict__ q1, unsigned int* status)
+{
+    constexpr int NV = LawTraits<LAW>::nv;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const double* R = res + c * (NV + 1);
+    const double fac = mode == 0 ? coef / vol[c] : coef / R[NV];
+    double qc[NV], qn[NV];
+    to_conserved<double, LAW, BASIS>(sp, q0 + c * NV, qc);
+    for (int v = 0; v < NV; ++v) qc[v] = qc[v] + R[v] * fac;
+    from_conserved<double, LAW, BASIS>(sp, qc, qn);
+    bool fin = true;
+    for (int v = 0; v < NV; ++v) {
+        q1[c * NV + v] = qn[v];
+        fin = fin && hc_isfinite(qn[v]);
+    }
+    if (!fin) atomicOr(status, ST_NONFINITE);
+}
+
+template <class F> hipError_t dispatch(int law, int basis, F&& f)
+{
+    if (law == HC_LAW_SCALAR) return f(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
+    if (basis == HC_BASIS_PRIMITIVE) return f(std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+    return f(std::integral_constant<int, 1>{}, std::integral_constant<int, 0>{});
+}
+
+unsigned blocks(long long n) { return (unsigned)((n + 255) / 256); }
+
+}   // namespace
+
+using namespace hc_detail;
+
+namespace {
+
+int single_case(hc_context* ctx)
+{
+    int r = case_ready(ctx, 0);
+    if (r) return r;
+    if (ctx->cases.size() != 1) return fail(ctx, HC_ERR_STATE, "the spatial building blocks act on a single-case context");
+    HC_HIP(hipSetDevice(ctx->device));
+    return HC_OK;
+}
+
+MeshView<double> mesh_view(const hc_context* ctx)
+{
+    const CaseHost& c = ctx->cases[0];
+    MeshView<double> m{};
+    m.ni = c.ni;
+    m.nj = c.nj;
+    m.per_i = c.cs.bc[0] == HC_BC_PERIODIC;
+    m.per_j = c.cs.bc[2] == HC_BC_PERIODIC;
+    m.nfi = c.nfi;
+    m.nfj = c.nfj;
+    m.cvol = ctx->cvol;
+    m.ccx = ctx->ccx;
+    m.ccy = ctx->ccy;
+    m.fi = ctx->fi;
+    m.fj = ctx->fj;
+    m.fi64 = ctx->fi;
+    m.fj64 = ctx->fj;
+    for (int d = 0; d < 2; ++d) {
+        m.period_i[d] = c.cs.period_i[d];
+        m.period_j[d] = c.cs.period_j[d];
+    }
+    return m;
+}
+
+Species<double> species_of(const hc_context* ctx)
+{
+    const hc_species& s = ctx->cases[0].cs.sp;
+    return Species<double>{s.gamma, s.minf, s.lref, s.nu, s.pr, s.dt, s.R, s.gamma1};
+}
+
+// device staging buffer of the building blocks (grown on demand, released with the case)
+int staging(hc_context* ctx, size_t ndoubles, double** out)
+{
+    if (ctx->blk_n < ndoubles) {
+        HC_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->blk) (void)hipFree(ctx->blk);
+        ctx->blk = nullptr;
+        ctx->blk_n = 0;
+        HC_HIP(hipMalloc(&ctx->blk, ndoubles * sizeof(double)));
+        ctx->blk_n = ndoubles;
+    }
+    *out = ctx->blk;
+    return HC_OK;
+}
+
+}   // namespace
+
+extern "C" {
+
+int hc_xmetrics(hc_context* ctx, double* X)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!X) return fail(ctx, HC_ERR_ARG, "null output");
+    const std::vector<double>& x = ctx->cases[0].X;
+    for (size_t k = 0; k < x.size(); ++k) X[k] = x[k];
+    return HC_OK;
+}
+
+int hc_qmetrics(hc_context* ctx, const double* q, double* Q)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!q || !Q) return fail(ctx, HC_ERR_ARG, "null array");
+    const long long N = ctx->cases[0].N;
+    const size_t nv = (size_t)ctx->nv;
+    double* buf = nullptr;
+    r = staging(ctx, 3 * nv * N, &buf);
+    if (r) return r;
+    double* dq = buf;
+    double* dQ = buf + nv * N;
+    HC_HIP(hipMemcpyAsync(dq, q, nv * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const MeshView<double> m = mesh_view(ctx);
+    const hipError_t e = dispatch(ctx->cs.law, ctx->cs.basis, [&](auto L, auto) {
+        hipLaunchKernelGGL((k_qmetrics<decltype(L)::value>), dim3(blocks(N)), dim3(256), 0, ctx->stream, m, dq, N, dQ);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return hipfail(ctx, e, "qmetrics kernel launch");
+    HC_HIP(hipMemcpyAsync(Q, dQ, 2 * nv * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+int hc_boundary_update(hc_context* ctx, const double* q, const double* bref, double* ghost)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!q || !bref || !ghost) return fail(ctx, HC_ERR_ARG, "null array");
+    const CaseHost& c = ctx->cases[0];
+    const long long N = c.N;
+    const size_t nv = (size_t)ctx->nv, rows = (size_t)c.brows;
+    double* buf = nullptr;
+    r = staging(ctx, nv * N + 2 * rows * nv, &buf);
+    if (r) return r;
+    double* dq = buf;
+    double* dref = buf + nv * N;
+    double* dghost = dref + rows * nv;
+    HC_HIP(hipMemcpyAsync(dq, q, nv * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HC_HIP(hipMemcpyAsync(dref, bref, rows * nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    // rows of periodic boundaries are not written: keep the caller's values
+    HC_HIP(hipMemcpyAsync(dghost, ghost, rows * nv * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const MeshView<double> m = mesh_view(ctx);
+    const Species<double> sp = species_of(ctx);
+    const int4 bc{c.cs.bc[0], c.cs.bc[1], c.cs.bc[2], c.cs.bc[3]};
+    const hipError_t e = dispatch(ctx->cs.law, ctx->cs.basis, [&](auto L, auto B) {
+        hipLaunchKernelGGL((k_boundary_update<decltype(L)::value, decltype(B)::value>), dim3(blocks((long long)rows)),
+                           dim3(256), 0, ctx->stream, m, sp, bc, dq, dref, dghost, ctx->status);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return hipfail(ctx, e, "boundary update kernel launch");
+    HC_HIP(hipMemcpyAsync(ghost, dghost, rows * nv * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return check_status(ctx);
+}
+
+int hc_spectral_radius(hc_context* ctx, const double* res, double* out)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!res || !out) return fail(ctx, HC_ERR_ARG, "null array");
+    const long long N = ctx->cases[0].N;
+    const size_t nv = (size_t)ctx->nv;
+    constexpr int NB = 1024;
+    double* buf = nullptr;
+    r = staging(ctx, (nv + 1) * N + NB + 1, &buf);
+    if (r) return r;
+    double* dres = buf;
+    double* part = buf + (nv + 1) * N;
+    HC_HIP(hipMemcpyAsync(dres, res, (nv + 1) * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_specrad_partials, dim3(NB), dim3(256), 0, ctx->stream, dres, ctx->cvol, N, (int)nv, part);
+    hipLaunchKernelGGL(k_specrad_final, dim3(1), dim3(256), 0, ctx->stream, part, NB, part + NB);
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipMemcpyAsync(out, part + NB, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+int hc_forward_update(hc_context* ctx, int mode, double coef, const double* res, const double* q0, double* q1)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!res || !q0 || !q1) return fail(ctx, HC_ERR_ARG, "null array");
+    if (mode != 0 && mode != 1) return fail(ctx, HC_ERR_ARG, "mode must be 0 (global) or 1 (local)");
+    const long long N = ctx->cases[0].N;
+    const size_t nv = (size_t)ctx->nv;
+    double* buf = nullptr;
+    r = staging(ctx, (nv + 1) * N + 2 * nv * N, &buf);
+    if (r) return r;
+    double* dres = buf;
+    double* dq0 = buf + (nv + 1) * N;
+    double* dq1 = dq0 + nv * N;
+    HC_HIP(hipMemcpyAsync(dres, res, (nv + 1) * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    HC_HIP(hipMemcpyAsync(dq0, q0, nv * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const Species<double> sp = species_of(ctx);
+    const hipError_t e = dispatch(ctx->cs.law, ctx->cs.basis, [&](auto L, auto B) {
+        hipLaunchKernelGGL((k_forward_update<decltype(L)::value, decltype(B)::value>), dim3(blocks(N)), dim3(256), 0,
+                           ctx->stream, mode, coef, sp, ctx->cvol, dres, dq0, N, dq1, ctx->status);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return hipfail(ctx, e, "forward update kernel launch");
+    HC_HIP(hipMemcpyAsync(q1, dq1, nv * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return check_status(ctx);
+}
+
+}   // extern "C"

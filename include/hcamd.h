@@ -177,10 +177,10 @@ int hc_get_profile(hc_context* ctx, double* out4);
 int hc_device_state(hc_context* ctx, double** dptr);
 
 /*
- * STREAM triad a[i] = b[i] + s*c[i] over three device arrays of n doubles,
- * repeated reps times; returns the attainable HBM bandwidth in GB/s (24 n bytes
- * per repetition).  Used as the attainable-bandwidth denominator and to
- * calibrate memory counters.  No counterpart in the reference.
+ * STREAM triad a[i] = b[i] + s*c[i] over three device arrays of n doubles (n >= 2), repeated
+ * reps times with 16-byte loads and stores; returns the attainable HBM bandwidth in GB/s
+ * (24 bytes per double of the even part of n per repetition).  Used as the attainable-bandwidth
+ * denominator and to calibrate memory counters.  No counterpart in the reference.
  */
 int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps);
 
@@ -243,6 +243,32 @@ int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_
                         double* minf_bar);
 int hc_tangent_steps(hc_context* ctx, int nsteps, const double* qdot0, const double* bref_dot, double minf_dot,
                      double* qdot_final);
+
+/*
+ * Spatial building blocks of a single-case context, for callers that run the time loop
+ * themselves on host arrays (the reference's integrate() body, timestepping/rungeKutta.h:97-164);
+ * hc_step runs the same arithmetic fused.  Arrays as above; the geometry, boundary types, basis
+ * and species are the context's.
+ *
+ * hc_xmetrics: the LSQ spatial metric per cell X [N][3] = {xx, xy, yy} (xmetrics,
+ *   spatial/lsqMetrics.h:20-62 with boundary_xmetrics :104-163; mirror points on non-periodic
+ *   sides, periodic images across periodic ones).
+ * hc_qmetrics: the LSQ solution metric Q [N][nvar][2] of the state q (qmetrics,
+ *   spatial/lsqMetrics.h:181-213, lsq/qmetric.ipp:48-70).
+ * hc_boundary_update: ghost values [2nj+2ni][nvar] of the non-periodic boundaries from q and the
+ *   reference values bref (boundaryUpdate, spatial/boundary/boundaryUpdate.h:28-48, 137-266); rows
+ *   of periodic boundaries are left as the caller passed them.
+ * hc_spectral_radius: max over cells of lambda / volume of a residual [N][nvar+1] (spectralRadius,
+ *   spatial/spectralRadius.h:19-48, initial value DBL_MIN).
+ * hc_forward_update: q1 = S(C(q0) + R coef / vol) with mode 0 (eulerForwardUpdateGlobal,
+ *   spatial/eulerForwardUpdate.h:38-92, coef = dt) or q1 = S(C(q0) + R coef / lambda) with mode 1
+ *   (eulerForwardUpdateLocal, :117-168, coef = cfl, lambda of the residual passed).
+ */
+int hc_xmetrics(hc_context* ctx, double* X);
+int hc_qmetrics(hc_context* ctx, const double* q, double* Q);
+int hc_boundary_update(hc_context* ctx, const double* q, const double* bref, double* ghost);
+int hc_spectral_radius(hc_context* ctx, const double* res, double* out);
+int hc_forward_update(hc_context* ctx, int mode, double coef, const double* res, const double* q0, double* q1);
 
 /* Library version string. */
 const char* hc_version(void);
