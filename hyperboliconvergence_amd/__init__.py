@@ -34,6 +34,7 @@ C_ABI_SYMBOLS = (
     "hc_set_tiles", "hc_get_tiles",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
+    "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing",
 )
 NTILECFG = 3
 
@@ -113,6 +114,9 @@ def library():
         "hc_boundary_update": (i, [P, P, P, P]),
         "hc_spectral_radius": (i, [P, P, P]),
         "hc_forward_update": (i, [P, i, d, P, P, P]),
+        "hc_gradient": (i, [P, P, P]),
+        "hc_smooth_residual": (i, [P, i, d, P]),
+        "hc_set_residual_smoothing": (i, [P, i, d]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -337,6 +341,23 @@ class Context:
         self._check(self._L.hc_forward_update(self._h, int(mode), float(coef), _p(res), _p(q0), _p(q1)),
                     "hc_forward_update")
         return q1
+
+    def gradient(self, q):
+        """Legacy neighbour-difference gradient [ni][nj][2][nv] of the state q."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros((self.ni, self.nj, 2, self.nv))
+        self._check(self._L.hc_gradient(self._h, _p(q), _p(out)), "hc_gradient")
+        return out
+
+    def smooth_residual(self, res, mode, alpha):
+        """Residual smoothing of res [ni][nj][nv+1] (flux part): mode 0 explicit, 1 implicit, 2 both."""
+        out = np.array(res, dtype=np.float64, copy=True, order="C")
+        self._check(self._L.hc_smooth_residual(self._h, int(mode), float(alpha), _p(out)), "hc_smooth_residual")
+        return out
+
+    def set_residual_smoothing(self, mode, alpha=1.0):
+        """Smooth every stage's accumulated residual in the time loop (mode -1 off, 0..2 as smooth_residual)."""
+        self._check(self._L.hc_set_residual_smoothing(self._h, int(mode), float(alpha)), "hc_set_residual_smoothing")
 
     def stream_triad(self, n, reps):
         g = C.c_double()

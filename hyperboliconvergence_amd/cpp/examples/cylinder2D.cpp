@@ -4,10 +4,11 @@
 // integrate, and par::for_each_idx writers of result.dat / mesh.dat.  Euler on an O-grid around a
 // unit-diameter cylinder, slip wall, Riemann far field, periodic theta, Roe flux with MinMod2 MUSCL,
 // classical RK4 with local time stepping on the GPU.
-//   usage: cylinder2D [nth nr nsteps [mixed [outdir [drop]]]]   -- prints the state checksum and
-//   timing; with outdir (or "-"), writes outdir/result.dat and outdir/mesh.dat in the columns and
-//   blocks of the reference driver, for its gnuplot scripts; with drop, marches to a steady state
-//   (SteadyTimeControls, controls.h:31-48) for at most nsteps
+//   usage: cylinder2D [nth nr nsteps [mixed [outdir [drop [smoothing]]]]]   -- prints the state
+//   checksum and timing; with outdir (or "-"), writes outdir/result.dat and outdir/mesh.dat in the
+//   columns and blocks of the reference driver, for its gnuplot scripts; with drop, marches to a
+//   steady state (SteadyTimeControls, controls.h:31-48) for at most nsteps, with implicit residual
+//   smoothing at that factor when smoothing > 1
 #include <hcamd/hcamd.hpp>
 
 #include <chrono>
@@ -97,8 +98,9 @@ int main(int argc, char** argv)
     // integrate forward in time
     const auto t0 = std::chrono::steady_clock::now();
     if (argc > 6) {
+        const Real factor = argc > 7 ? std::atof(argv[7]) : 1.0;
         const SteadyTimeControls<Real> steady{.nTimesteps = nt, .cfl = 1.0, .residual_drop = std::atof(argv[6]),
-                                              .use_residual_smoothing = false, .smoothing_factor = 1.0};
+                                              .use_residual_smoothing = factor > 1.0, .smoothing_factor = factor};
         const Real drop = integrate(policy, steady, rk, hoflux, boundaryConditions, species, mesh, q);
         std::printf("steady: residual drop %.6e\n", drop);
     } else {

@@ -1216,6 +1216,46 @@ void eulerForwardUpdateLocal(const Policy policy, const Species<Law, Real>& spec
         q1, q0, r);
 }
 
+/*
+ * gradientCalc(policy, mesh, q[, dq]) (spatial/gradientCalc.h:16-313): per cell and direction the
+ * sum of the neighbour differences q_R - q_L over its edges, the periodic wraps stopping one short
+ * of the last row / column like the reference's loops; on the GPU.
+ */
+template <typename Policy, int nDim, floating_point Real, typename VarSetT>
+void gradientCalc(const Policy policy, const Mesh<nDim, Real>& mesh, const SolutionField<VarSetT, nDim>& q,
+                  par::DualArray<std::array<vardelta_t<VarSetT>, 2>, 2>& dq)
+{
+    static_assert(nDim == 2, "two-dimensional meshes");
+    constexpr LawType Law = law_of_v<VarSetT>;
+    assert(mesh.cells.shape() == q.interior.shape() && mesh.cells.shape() == dq.shape());
+    const auto rk = ODE::Explicit::ssp11<Real>();
+    double alpha[6][6], beta[6];
+    hcamd_detail::rk_arrays(rk, alpha, beta);
+    // the boundary kinds matter only through their periodicity here
+    std::tuple<BoundaryCondition<Law, BoundaryType<Law>::Periodic>, BoundaryCondition<Law, BoundaryType<Law>::Fixed>> kinds;
+    std::array<BoundaryType<Law>, 4> types;
+    for (int b = 0; b < 4; ++b)
+        types[b] = q.bcTypes[b] == BoundaryType<Law>::Periodic ? BoundaryType<Law>::Periodic : BoundaryType<Law>::Fixed;
+    Species<Law, Real> sp{};
+    if constexpr (Law == LawType::Euler) sp = get_air_species<Real>();
+    const hc_case_desc d = hcamd_detail::describe<Law, varset_traits<VarSetT>::basis>(
+        HC_FLUX_CENTRAL, 0, 0, HC_LIM_NOLIMIT1, kinds, types, sp, mesh, 1, alpha, beta, 1.0, HC_DT_GLOBAL);
+    const auto nodes = hcamd_detail::flat_nodes(mesh);
+    hc_context* ctx =
+        hcamd_detail::session(hcamd_detail::hip_of(policy), nodes.data(), mesh.cells.shape(0), mesh.cells.shape(1), d);
+    static_assert(sizeof(std::array<vardelta_t<VarSetT>, nDim>) == sizeof(double) * nDim * VarSetT::N, "packed deltas");
+    hcamd_detail::check(hc_gradient(ctx, hcamd_detail::raw(q.interior), reinterpret_cast<double*>(dq.data())), ctx,
+                        "hc_gradient");
+}
+template <typename Policy, int nDim, floating_point Real, typename VarSetT>
+par::DualArray<std::array<vardelta_t<VarSetT>, 2>, 2> gradientCalc(const Policy policy, const Mesh<nDim, Real>& mesh,
+                                                                  const SolutionField<VarSetT, nDim>& q)
+{
+    par::DualArray<std::array<vardelta_t<VarSetT>, 2>, 2> dq(q.interior.shape());
+    gradientCalc(policy, mesh, q, dq);
+    return dq;
+}
+
 // ---------------- time integration on the GPU ----------------
 
 namespace hcamd_detail {
@@ -1242,6 +1282,7 @@ hc_context* integrate_session(const Policy policy, const ODE::Explicit::RungeKut
     check(hc_set_state(ctx, raw(q0.interior)), ctx, "hc_set_state");
     unsigned int status = 0;
     check(hc_get_status(ctx, &status), ctx, "hc_get_status");   // clear what earlier calls left
+    check(hc_set_residual_smoothing(ctx, -1, 1.0), ctx, "hc_set_residual_smoothing");
     return ctx;
 }
 }   // namespace hcamd_detail
@@ -1285,7 +1326,9 @@ Real integrate(const UnsteadyTimeControls<Real>& timeControls, const ODE::Explic
  * Steady-state integrate with SteadyTimeControls (controls.h:31-48, no integrate consumes it at the
  * reference's HEAD): marches until the mass residual's L2 norm has dropped by
  * timeControls.residual_drop (checked every 50 steps) or nTimesteps are taken, writes the state
- * back into q0 and returns the drop reached.  Residual smoothing is not built.
+ * back into q0 and returns the drop reached.  use_residual_smoothing smooths every stage's
+ * accumulated residual with the implicit operator of OLD.fieldOperations/centralImplicitSmoothing.ipp
+ * (along i, then along j) at alpha = smoothing_factor, which multiplies the stable time step.
  */
 template <typename Policy, LawType Law, int nDim, floating_point Real, typename SolVarT, typename SecondOrderFlux,
           typename... BoundaryConds>
@@ -1295,9 +1338,11 @@ Real integrate(const Policy policy, const SteadyTimeControls<Real>& timeControls
                const std::tuple<BoundaryConds...> boundaryConds, const Species<Law, Real>& species,
                const Mesh<nDim, Real>& mesh, SolutionField<SolVarT, nDim>& q0)
 {
-    if (timeControls.use_residual_smoothing) throw std::runtime_error("residual smoothing is not built");
     hc_context* ctx = hcamd_detail::integrate_session(policy, rungeKutta, timeControls.cfl, flux2, boundaryConds,
                                                       species, mesh, q0);
+    if (timeControls.use_residual_smoothing)
+        hcamd_detail::check(hc_set_residual_smoothing(ctx, 1, timeControls.smoothing_factor), ctx,
+                            "hc_set_residual_smoothing");
     long long steps = 0;
     double drop = 1.0;
     hcamd_detail::check(hc_step_steady(ctx, (long long)timeControls.nTimesteps, timeControls.residual_drop, 50, &steps,

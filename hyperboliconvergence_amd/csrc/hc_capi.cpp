@@ -194,6 +194,10 @@ void release_buffers(hc_context* ctx)
     ctx->nchunks = 0;
     free_dev(ctx->blk);
     ctx->blk_n = 0;
+    free_dev(ctx->sm);
+    ctx->sm_n = 0;
+    ctx->smooth_mode = -1;
+    ctx->smooth_alpha = 1.0;
     ctx->have_case = false;
     ctx->tuned = false;
     release_adjoint(ctx);
@@ -991,6 +995,14 @@ int hc_step(hc_context* ctx, int nsteps)
     r = all_states(ctx);
     if (r) return r;
     int done = 0;
+    if (ctx->smooth_mode >= 0) {
+        // residual smoothing: the unfused step (no graphs, the tile shape in use)
+        for (; done < nsteps; ++done) {
+            r = enqueue_step_smoothed(ctx);
+            if (r) return r;
+        }
+        return HC_OK;
+    }
     hipEvent_t r0 = nullptr;
     if (ctx->profiling == 2 && nsteps > 0) {
         r0 = take_event(ctx);

@@ -96,6 +96,11 @@ struct hc_context {
     // device staging buffer of the spatial building blocks (hc_qmetrics ... hc_forward_update)
     double* blk = nullptr;
     size_t blk_n = 0;
+    // residual smoothing of the stepping (hc_set_residual_smoothing): mode -1 off, its buffers
+    int smooth_mode = -1;
+    double smooth_alpha = 1.0;
+    double* sm = nullptr;
+    size_t sm_n = 0;
 };
 
 
@@ -110,6 +115,9 @@ int enqueue_step(hc_context* ctx);
 hipError_t launch_stage(hc_context* ctx, const hc::LaunchArgs<double>& L);
 template <class RF> hc::LaunchArgs<RF> launch_args(hc_context* ctx);
 void release_adjoint(hc_context* ctx);
+int smooth_device(hc_context* ctx, int mode, double alpha, double* r, double* b, double* c, int stride);
+int enqueue_step_smoothed(hc_context* ctx);
+void drop_graph(hc_context* ctx);
 
 }   // namespace hc_detail
 

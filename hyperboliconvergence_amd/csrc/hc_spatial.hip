@@ -163,6 +163,176 @@ __global__ void __launch_bounds__(256) k_forward_update(int mode, double coef, S
     if (!fin) atomicOr(status, ST_NONFINITE);
 }
 
+// legacy gradient (spatial/gradientCalc.h:16-313): per cell and direction the neighbour differences
+// q_R - q_L of its edges in the reference's order (the i-edge to the lower neighbour, then to the
+// upper one; the same along j), then the periodic wraps, whose loops stop one short of the last
+// row / column (j < nj-1 for the i-wrap, i < ni-1 for the j-wrap).  dq [N][2][nv].
+template <int LAW>
+__global__ void __launch_bounds__(256) k_gradient(int ni, int nj, int per_i, int per_j, const double* __restrict__ q,
+                                                  double* __restrict__ dq)
+{
+    constexpr int NV = LawTraits<LAW>::nv;
+    const long long N = (long long)ni * nj;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const int i = (int)(c / nj), j = (int)(c % nj);
+    for (int v = 0; v < NV; ++v) {
+        const double qc = q[c * NV + v];
+        double d0 = 0.0, d1 = 0.0;
+        if (i > 0) d0 = d0 + (qc - q[(c - nj) * NV + v]);
+        if (i < ni - 1) d0 = d0 + (q[(c + nj) * NV + v] - qc);
+        if (j > 0) d1 = d1 + (qc - q[(c - 1) * NV + v]);
+        if (j < nj - 1) d1 = d1 + (q[(c + 1) * NV + v] - qc);
+        if (per_i && j < nj - 1 && (i == 0 || i == ni - 1))
+            d0 = d0 + (q[(long long)j * NV + v] - q[((long long)(ni - 1) * nj + j) * NV + v]);
+        if (per_j && i < ni - 1 && (j == 0 || j == nj - 1))
+            d1 = d1 + (q[((long long)i * nj) * NV + v] - q[((long long)i * nj + nj - 1) * NV + v]);
+        dq[(c * 2 + 0) * NV + v] = d0;
+        dq[(c * 2 + 1) * NV + v] = d1;
+    }
+}
+
+// Residual smoothing, the 1-D operators of OLD.fieldOperations/centralExplicitSmoothing.ipp:5-39
+// and centralImplicitSmoothing.ipp:2-60 along the lines of one direction of the field
+// r[N][stride] (values of variable v at r[c * stride + v]); dir 0 along i, 1 along j.
+struct Line {
+    int n;                 // points on the line
+    long long step;        // cell-index step between neighbours on the line
+    long long c;           // cell index of this point
+    int k;                 // position on the line
+};
+__device__ inline Line line_of(long long c, int ni, int nj, int dir)
+{
+    const int i = (int)(c / nj), j = (int)(c % nj);
+    if (dir == 0) return Line{ni, (long long)nj, c, i};
+    return Line{nj, 1LL, c, j};
+}
+
+// explicit: r1[k] = (r0[k] + gamma (r0[k] - r0[k-1])) - gamma (r0[k+1] - r0[k]); on a periodic line
+// the wrap edge (n-1, 0) adds to the end points last
+__global__ void __launch_bounds__(256) k_smooth_explicit(int ni, int nj, int dir, int periodic, int nv, int stride,
+                                                         double gamma, const double* __restrict__ r0,
+                                                         double* __restrict__ r1)
+{
+    const long long N = (long long)ni * nj;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const Line L = line_of(c, ni, nj, dir);
+    const long long first = c - (long long)L.k * L.step, last = first + (long long)(L.n - 1) * L.step;
+    for (int v = 0; v < nv; ++v) {
+        auto at = [&](long long cc) { return r0[cc * stride + v]; };
+        const double x = at(c);
+        double y = x;
+        if (L.k > 0) y = y + gamma * (x - at(c - L.step));
+        if (L.k < L.n - 1) y = y - gamma * (at(c + L.step) - x);
+        if (periodic) {
+            const double dr = gamma * (at(first) - at(last));
+            if (L.k == L.n - 1) y = y - dr;
+            if (L.k == 0) y = y + dr;
+        }
+        r1[c * stride + v] = y;
+    }
+}
+
+// one weighted Jacobi sweep of (1 + 2 beta) r - beta (r[k-1] + r[k+1]) = r0:
+//   s = (r0[k] + beta w[k-1]) + beta w[k+1], scaled by 1/(1 + 2 beta) (end points of a non-periodic
+//   line: 1/(1 + beta)), w' = (1 - omega) w + omega s; the last sweep also multiplies by alpha
+__global__ void __launch_bounds__(256) k_smooth_jacobi(int ni, int nj, int dir, int periodic, int nv, int stride,
+                                                       double beta, double omega, double alpha, int last_sweep,
+                                                       const double* __restrict__ r0, const double* __restrict__ w,
+                                                       double* __restrict__ wn)
+{
+    const long long N = (long long)ni * nj;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const Line L = line_of(c, ni, nj, dir);
+    const long long first = c - (long long)L.k * L.step, last = first + (long long)(L.n - 1) * L.step;
+    const double s2 = 1. / (1. + 2. * beta), s1 = 1. / (1. + 1. * beta);
+    for (int v = 0; v < nv; ++v) {
+        double s = r0[c * stride + v];
+        if (L.k > 0) s = s + beta * w[(c - L.step) * stride + v];
+        if (L.k < L.n - 1) s = s + beta * w[(c + L.step) * stride + v];
+        const bool end = L.k == 0 || L.k == L.n - 1;
+        if (!end) s = s * s2;
+        else if (periodic) {
+            s = s + beta * w[(L.k == 0 ? last : first) * stride + v];
+            s = s * s2;
+        } else {
+            s = s * s1;
+        }
+        double y = (1. - omega) * w[c * stride + v] + omega * s;
+        if (last_sweep) y = y * alpha;
+        wn[c * stride + v] = y;
+    }
+}
+
+// ---- time step with residual smoothing (stepping of a single-case context) ----
+
+// dt = cfl / max(part), dtslot[0] = dt, dtslot[1] += dt (the global step of stage 0, rungeKutta.h:133)
+__global__ void __launch_bounds__(256) k_dt_from_max(const double* part, int nparts, double cfl, double* dtslot)
+{
+    __shared__ double sm[256];
+    double m = DBL_MIN;
+    for (int p = threadIdx.x; p < nparts; p += 256) m = fmax(m, part[p]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dt = cfl / sm[0];
+        dtslot[0] = dt;
+        dtslot[1] = dtslot[1] + dt;
+    }
+}
+
+struct AlphaRow {
+    double a[6];
+};
+
+// R_tot = sum_{k <= s} alpha[s][k] R_k in that order (rungeKutta.h:176-197); its lambda column holds
+// stage 0's spectral-radius sum (the local time step's lambda)
+template <int NV>
+__global__ void __launch_bounds__(256) k_rk_total(int s, AlphaRow al, const double* __restrict__ R, long long N,
+                                                  double* __restrict__ Rt)
+{
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    for (int v = 0; v < NV; ++v) {
+        double t = 0.;
+        for (int k = 0; k <= s; ++k) t = t + R[((long long)k * N + c) * (NV + 1) + v] * al.a[k];
+        Rt[c * (NV + 1) + v] = t;
+    }
+    Rt[c * (NV + 1) + NV] = R[c * (NV + 1) + NV];
+}
+
+// q_out = S(C(q0) + R_tot fac), fac = (beta dt) / vol (global) or (beta cfl) / lambda_0 (local),
+// on the device's [nv][N] state planes
+template <int LAW, int BASIS>
+__global__ void __launch_bounds__(256) k_update_planes(int global, double beta, double cfl, const double* dtslot,
+                                                       Species<double> sp, const double* __restrict__ Rt,
+                                                       const double* __restrict__ vol, const double* q0, long long N,
+                                                       double* qout, unsigned int* status)
+{
+    constexpr int NV = LawTraits<LAW>::nv;
+    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (c >= N) return;
+    const double* R = Rt + c * (NV + 1);
+    const double fac = global ? (beta * dtslot[0]) / vol[c] : (beta * cfl) / R[NV];
+    double q[NV], qc[NV], qn[NV];
+    for (int v = 0; v < NV; ++v) q[v] = q0[v * N + c];
+    to_conserved<double, LAW, BASIS>(sp, q, qc);
+    for (int v = 0; v < NV; ++v) qc[v] = qc[v] + R[v] * fac;
+    from_conserved<double, LAW, BASIS>(sp, qc, qn);
+    bool fin = true;
+    for (int v = 0; v < NV; ++v) {
+        qout[v * N + c] = qn[v];
+        fin = fin && hc_isfinite(qn[v]);
+    }
+    if (!fin) atomicOr(status, ST_NONFINITE);
+}
+
 template <class F> hipError_t dispatch(int law, int basis, F&& f)
 {
     if (law == HC_LAW_SCALAR) return f(std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
@@ -233,6 +403,121 @@ int staging(hc_context* ctx, size_t ndoubles, double** out)
 }
 
 }   // namespace
+
+
+namespace hc_detail {
+
+// Residual smoothing of the flux part of r [N][stride] on the device (mode 0 explicit, 1 implicit,
+// 2 explicit then implicit; each operator along i, then along j); b and c are scratch buffers of
+// r's size; the result is left in r, the lambda column unchanged.  Enqueued on the context's stream.
+int smooth_device(hc_context* ctx, int mode, double alpha, double* r, double* b, double* c, int stride)
+{
+    const CaseHost& cs = ctx->cases[0];
+    const long long N = cs.N;
+    const int nv = ctx->nv;
+    const int per[2] = {cs.cs.bc[0] == HC_BC_PERIODIC, cs.cs.bc[2] == HC_BC_PERIODIC};
+    const double beta = 0.25 * (alpha * alpha - 1.);
+    const double gamma = 0.25 * ((1. + 4. * beta) / alpha - 1.);
+    const double omega = 2. / 3.;
+    const unsigned nb = blocks(N);
+    const size_t bytes = (size_t)stride * N * sizeof(double);
+    // the scratch buffers carry the lambda column along
+    HC_HIP(hipMemcpyAsync(b, r, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HC_HIP(hipMemcpyAsync(c, r, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const int first = mode == 1 ? 1 : 0, last = mode == 0 ? 0 : 1;
+    for (int m = first; m <= last; ++m)
+        for (int dir = 0; dir < 2; ++dir) {
+            if (m == 0) {
+                hipLaunchKernelGGL(k_smooth_explicit, dim3(nb), dim3(256), 0, ctx->stream, cs.ni, cs.nj, dir, per[dir],
+                                   nv, stride, gamma, r, b);
+                HC_HIP(hipGetLastError());
+                HC_HIP(hipMemcpyAsync(r, b, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            } else {
+                // r holds r0 throughout; the iterates alternate r0 -> b -> c -> b -> c
+                const double* w = r;
+                for (int it = 0; it < 4; ++it) {
+                    double* wn = (it % 2 == 0) ? b : c;
+                    hipLaunchKernelGGL(k_smooth_jacobi, dim3(nb), dim3(256), 0, ctx->stream, cs.ni, cs.nj, dir,
+                                       per[dir], nv, stride, beta, omega, alpha, it == 3 ? 1 : 0, r, w, wn);
+                    HC_HIP(hipGetLastError());
+                    w = wn;
+                }
+                HC_HIP(hipMemcpyAsync(r, c, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            }
+        }
+    return HC_OK;
+}
+
+
+// One time step of a single-case context with its stage residuals smoothed: per stage the residual
+// of the stage state (the stage kernel in residual mode), the global time step at stage 0, the
+// Runge-Kutta accumulation, the smoothing of the accumulated residual, and the forward-Euler update
+// from the step-start state.  With the explicit operator at alpha = 1 (gamma = 0) this is the
+// fused step bit for bit.  Enqueued on the context's stream.
+int enqueue_step_smoothed(hc_context* ctx)
+{
+    const CaseHost& cs = ctx->cases[0];
+    const long long N = cs.N;
+    const int nv = ctx->nv, ns = ctx->cs.nstages;
+    const size_t blk = (size_t)(nv + 1) * N;
+    constexpr int NB = 1024;
+    const size_t need = (size_t)(ns + 3) * blk + NB;
+    if (ctx->sm_n < need) {
+        HC_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->sm) (void)hipFree(ctx->sm);
+        ctx->sm = nullptr;
+        ctx->sm_n = 0;
+        HC_HIP(hipMalloc(&ctx->sm, need * sizeof(double)));
+        ctx->sm_n = need;
+    }
+    double* R = ctx->sm;
+    double* Rt = R + (size_t)ns * blk;
+    double* s1 = Rt + blk;
+    double* s2 = s1 + blk;
+    double* part = s2 + blk;
+    const bool global = ctx->cs.dt_mode == HC_DT_GLOBAL;
+    double* q0 = ctx->q[ctx->cur];
+    double* bufA = ns == 1 ? ctx->q[1 - ctx->cur] : ctx->q[(ctx->cur + 1) % 3];
+    double* bufB = ns == 1 ? bufA : ctx->q[(ctx->cur + 2) % 3];
+    const Species<double> sp = species_of(ctx);
+    const unsigned nb = blocks(N);
+    const double* in = q0;
+    for (int s = 0; s < ns; ++s) {
+        double* out = ns == 1 ? bufA : (s == ns - 1 ? q0 : (s % 2 == 0 ? bufA : bufB));
+        LaunchArgs<double> L = launch_args<double>(ctx);
+        L.mode = MODE_RESIDUAL;
+        L.q_in = in;
+        L.q0 = in;
+        L.q_out = out;   // not written in residual mode
+        L.res_out = R + (size_t)s * blk;
+        L.plan = ctx->plans[0];
+        hipError_t e = launch_stage(ctx, L);
+        if (e != hipSuccess) return hipfail(ctx, e, "residual kernel launch");
+        if (s == 0 && global) {
+            hipLaunchKernelGGL(k_specrad_partials, dim3(NB), dim3(256), 0, ctx->stream, R, ctx->cvol, N, nv, part);
+            hipLaunchKernelGGL(k_dt_from_max, dim3(1), dim3(256), 0, ctx->stream, part, NB, ctx->cs.cfl, ctx->dtslot);
+        }
+        AlphaRow al{};
+        for (int k = 0; k < 6; ++k) al.a[k] = ctx->cs.alpha[s][k];
+        if (nv == 4) hipLaunchKernelGGL(k_rk_total<4>, dim3(nb), dim3(256), 0, ctx->stream, s, al, R, N, Rt);
+        else hipLaunchKernelGGL(k_rk_total<3>, dim3(nb), dim3(256), 0, ctx->stream, s, al, R, N, Rt);
+        HC_HIP(hipGetLastError());
+        const int r = smooth_device(ctx, ctx->smooth_mode, ctx->smooth_alpha, Rt, s1, s2, nv + 1);
+        if (r) return r;
+        e = dispatch(ctx->cs.law, ctx->cs.basis, [&](auto Lw, auto B) {
+            hipLaunchKernelGGL((k_update_planes<decltype(Lw)::value, decltype(B)::value>), dim3(nb), dim3(256), 0,
+                               ctx->stream, global ? 1 : 0, ctx->cs.beta[s], ctx->cs.cfl, ctx->dtslot, sp, Rt, ctx->cvol,
+                               q0, N, out, ctx->status);
+            return hipGetLastError();
+        });
+        if (e != hipSuccess) return hipfail(ctx, e, "update kernel launch");
+        in = out;
+    }
+    if (ns == 1) ctx->cur = 1 - ctx->cur;
+    return HC_OK;
+}
+
+}   // namespace hc_detail
 
 extern "C" {
 
@@ -350,6 +635,67 @@ int hc_forward_update(hc_context* ctx, int mode, double coef, const double* res,
     HC_HIP(hipMemcpyAsync(q1, dq1, nv * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HC_HIP(hipStreamSynchronize(ctx->stream));
     return check_status(ctx);
+}
+
+int hc_gradient(hc_context* ctx, const double* q, double* dq)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!q || !dq) return fail(ctx, HC_ERR_ARG, "null array");
+    const CaseHost& c = ctx->cases[0];
+    const long long N = c.N;
+    const size_t nv = (size_t)ctx->nv;
+    double* buf = nullptr;
+    r = staging(ctx, 3 * nv * N, &buf);
+    if (r) return r;
+    double* dqin = buf;
+    double* dout = buf + nv * N;
+    HC_HIP(hipMemcpyAsync(dqin, q, nv * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const int per_i = c.cs.bc[0] == HC_BC_PERIODIC, per_j = c.cs.bc[2] == HC_BC_PERIODIC;
+    const hipError_t e = dispatch(ctx->cs.law, ctx->cs.basis, [&](auto L, auto) {
+        hipLaunchKernelGGL((k_gradient<decltype(L)::value>), dim3(blocks(N)), dim3(256), 0, ctx->stream, c.ni, c.nj,
+                           per_i, per_j, dqin, dout);
+        return hipGetLastError();
+    });
+    if (e != hipSuccess) return hipfail(ctx, e, "gradient kernel launch");
+    HC_HIP(hipMemcpyAsync(dq, dout, 2 * nv * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+int hc_smooth_residual(hc_context* ctx, int mode, double alpha, double* res)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!res) return fail(ctx, HC_ERR_ARG, "null array");
+    if (mode < 0 || mode > 2 || !(alpha >= 1.0)) return fail(ctx, HC_ERR_ARG, "smoothing mode 0..2 and alpha >= 1");
+    const long long N = ctx->cases[0].N;
+    const int stride = ctx->nv + 1;
+    double* buf = nullptr;
+    r = staging(ctx, 3 * (size_t)stride * N, &buf);
+    if (r) return r;
+    HC_HIP(hipMemcpyAsync(buf, res, (size_t)stride * N * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    r = smooth_device(ctx, mode, alpha, buf, buf + (size_t)stride * N, buf + 2 * (size_t)stride * N, stride);
+    if (r) return r;
+    HC_HIP(hipMemcpyAsync(res, buf, (size_t)stride * N * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+int hc_set_residual_smoothing(hc_context* ctx, int mode, double alpha)
+{
+    if (!ctx) return HC_ERR_ARG;
+    if (mode < -1 || mode > 2 || (mode >= 0 && !(alpha >= 1.0)))
+        return fail(ctx, HC_ERR_ARG, "smoothing mode -1 (off) or 0..2 with alpha >= 1");
+    if (mode >= 0) {
+        int r = single_case(ctx);
+        if (r) return r;
+        if (ctx->cs.dt_mode == HC_DT_FIXED) return fail(ctx, HC_ERR_ARG, "residual smoothing needs a global or local time step");
+    }
+    drop_graph(ctx);
+    ctx->smooth_mode = mode;
+    ctx->smooth_alpha = mode >= 0 ? alpha : 1.0;
+    return HC_OK;
 }
 
 }   // extern "C"

@@ -270,6 +270,31 @@ int hc_boundary_update(hc_context* ctx, const double* q, const double* bref, dou
 int hc_spectral_radius(hc_context* ctx, const double* res, double* out);
 int hc_forward_update(hc_context* ctx, int mode, double coef, const double* res, const double* q0, double* q1);
 
+/*
+ * hc_gradient: the legacy neighbour-difference gradient dq [N][2][nvar] of the state q
+ *   (gradientCalc, spatial/gradientCalc.h:16-313): per direction the sum of q_R - q_L over the
+ *   cell's edges, the periodic wraps stopping one short of the last row / column like the
+ *   reference's loops (:275, :296).
+ * hc_smooth_residual: residual smoothing of res [N][nvar+1] in place (the flux part), the product
+ *   of the reference's 1-D operators along i then along j (OLD.fieldOperations/
+ *   centralExplicitSmoothing.ipp:5-39, centralImplicitSmoothing.ipp:2-60): mode 0 explicit
+ *   (gamma = ((1 + 4 beta)/alpha - 1)/4), 1 implicit (4 weighted Jacobi sweeps, beta =
+ *   (alpha^2 - 1)/4, times alpha), 2 explicit then implicit; alpha >= 1 is the time-step factor.
+ *   A direction is periodic when its boundaries are.
+ */
+int hc_gradient(hc_context* ctx, const double* q, double* dq);
+int hc_smooth_residual(hc_context* ctx, int mode, double alpha, double* res);
+
+/*
+ * Residual smoothing in the time loop (SteadyTimeControls::use_residual_smoothing and
+ * smoothing_factor, controls.h:31-48, which no integrate() consumes at the reference's HEAD):
+ * mode 0..2 and alpha as hc_smooth_residual, applied to every stage's accumulated residual
+ * before its update in hc_step / hc_step_steady (an unfused step: residual, accumulation,
+ * smoothing and update as separate kernels); mode -1 switches it off.  Single-case contexts with
+ * a global or local time step.
+ */
+int hc_set_residual_smoothing(hc_context* ctx, int mode, double alpha);
+
 /* Library version string. */
 const char* hc_version(void);
 

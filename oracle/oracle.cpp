@@ -1599,8 +1599,15 @@ double orc_spectral_radius(int ni, int nj, const double* cells, const double* re
 }
 
 // timestepping/rungeKutta.h:37-167 and :172-197, spatial/eulerForwardUpdate.h:38-92
-int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, double* q0,
-                  const double* bref, int nsteps, double* t_out, double* timers_ms)
+void orc_smooth_residual(const orc_case* cs, int ni, int nj, int mode, double alpha, double* res);
+
+/*
+ * integrate() (timestepping/rungeKutta.h:37-167); smooth_mode >= 0 smooths the accumulated
+ * stage residual (orc_smooth_residual) before the forward-Euler update, the residual smoothing of
+ * SteadyTimeControls (controls.h:31-48), which no integrate() of the reference consumes.
+ */
+static int integrate_impl(const orc_case* cs, int ni, int nj, const double* nodes, double* q0, const double* bref,
+                          int nsteps, double* t_out, double* timers_ms, int smooth_mode, double smooth_alpha)
 {
     if (!valid_case(*cs, ni, nj)) return -1;
     using clk = std::chrono::steady_clock;
@@ -1641,6 +1648,16 @@ int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, doubl
                 for (int k = 0; k <= stg; ++k)
                     for (int v = 0; v < nv; ++v) rt[v] += rs[k][c].f[v] * alpha[k];
             }
+            if (smooth_mode >= 0) {
+                std::vector<double> r(N * (nv + 1));
+                for (size_t c = 0; c < N; ++c) {
+                    for (int v = 0; v < nv; ++v) r[c * (nv + 1) + v] = resTotal[c * nv + v];
+                    r[c * (nv + 1) + nv] = rs[0][c].lambda;
+                }
+                orc_smooth_residual(cs, ni, nj, smooth_mode, smooth_alpha, r.data());
+                for (size_t c = 0; c < N; ++c)
+                    for (int v = 0; v < nv; ++v) resTotal[c * nv + v] = r[c * (nv + 1) + v];
+            }
             lap(4, t0);
             const bool by_vol = cs->dt_mode != ORC_DT_LOCAL;   // global and fixed: (beta dt) / V
             const double stepfac = by_vol ? cs->beta[stg] * dt : cs->beta[stg] * cs->cfl;
@@ -1664,6 +1681,18 @@ int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, doubl
     if (t_out) *t_out = t;
     if (timers_ms) for (int k = 0; k < 7; ++k) timers_ms[k] = tm[k];
     return F.bc_ok ? 0 : -2;
+}
+
+int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, double* q0,
+                  const double* bref, int nsteps, double* t_out, double* timers_ms)
+{
+    return integrate_impl(cs, ni, nj, nodes, q0, bref, nsteps, t_out, timers_ms, -1, 1.0);
+}
+
+int orc_integrate_smoothed(const orc_case* cs, int ni, int nj, const double* nodes, double* q0, const double* bref,
+                           int nsteps, int smooth_mode, double smooth_alpha, double* t_out)
+{
+    return integrate_impl(cs, ni, nj, nodes, q0, bref, nsteps, t_out, nullptr, smooth_mode, smooth_alpha);
 }
 
 void orc_norms(int ni, int nj, int nvar, const double* cells, const double* res, double* l2, double* linf)
@@ -1729,6 +1758,107 @@ void orc_rk_table(int which, int* nstages, double* alpha36, double* beta6)
         beta6[s] = b[s];
         for (int k = 0; k < 6; ++k) alpha36[6 * s + k] = a[s][k];
     }
+}
+
+/*
+ * Legacy gradient (spatial/gradientCalc.h:16-313): per cell and direction the sum of the
+ * neighbour differences q_R - q_L over its edges (each edge added to both of its cells: i-edges
+ * into dq[0], j-edges into dq[1], :153-203), then the periodic wraps (:247-313), whose loops stop
+ * one short of the last row / column (j < nj-1, i < ni-1; quirk A-Q8).  dq [N][2][nvar].
+ */
+void orc_gradient(const orc_case* cs, int ni, int nj, const double* q, double* dq)
+{
+    const int nv = nvar_of(cs->law);
+    const size_t N = (size_t)ni * nj;
+    for (size_t k = 0; k < N * 2 * nv; ++k) dq[k] = 0.;
+    auto add = [&](size_t cl, size_t cr, int dir) {
+        for (int v = 0; v < nv; ++v) {
+            const double d = q[cr * nv + v] - q[cl * nv + v];
+            dq[(cl * 2 + dir) * nv + v] += d;
+            dq[(cr * 2 + dir) * nv + v] += d;
+        }
+    };
+    for (int j = 0; j < nj; ++j)
+        for (int i = 0; i < ni - 1; ++i) add((size_t)i * nj + j, (size_t)(i + 1) * nj + j, 0);
+    for (int i = 0; i < ni; ++i)
+        for (int j = 0; j < nj - 1; ++j) add((size_t)i * nj + j, (size_t)i * nj + j + 1, 1);
+    if (cs->bc[0] == ORC_BC_PERIODIC)
+        for (int j = 0; j < nj - 1; ++j) add((size_t)(ni - 1) * nj + j, (size_t)j, 0);
+    if (cs->bc[2] == ORC_BC_PERIODIC)
+        for (int i = 0; i < ni - 1; ++i) add((size_t)i * nj + nj - 1, (size_t)i * nj, 1);
+}
+
+/*
+ * Residual smoothing along one grid line of n values with stride `stride` (the reference's 1-D
+ * operators, OLD.fieldOperations/centralExplicitSmoothing.ipp:5-39 and
+ * centralImplicitSmoothing.ipp:2-60; beta = (alpha^2 - 1)/4, gamma = ((1 + 4 beta)/alpha - 1)/4):
+ *   explicit: r1 = r0; for each edge (i-1, i): dr = gamma (r0[i] - r0[i-1]), r1[i-1] -= dr, r1[i] += dr,
+ *             plus the wrap edge (n-1, 0) on a periodic line;
+ *   implicit: 4 weighted (omega = 2/3) Jacobi sweeps of (1 + 2 beta) r - beta (r[i-1] + r[i+1]) = r0
+ *             from r0 (the end values of a non-periodic line divide by 1 + beta), times alpha.
+ * periodic: the line closes on itself ('p'), else the reference's 'a' ends.
+ */
+static void smooth_line(int mode, double alpha, bool periodic, int n, double* r, size_t stride)
+{
+    const double beta = 0.25 * (alpha * alpha - 1.);
+    const double gamma = 0.25 * ((1. + 4. * beta) / alpha - 1.);
+    std::vector<double> r0(n), r1(n), wrk(n);
+    for (int i = 0; i < n; ++i) r0[i] = r[i * stride];
+    if (mode == 0) {
+        r1 = r0;
+        for (int i = 1; i < n; ++i) {
+            const double dr = gamma * (r0[i] - r0[i - 1]);
+            r1[i - 1] -= dr;
+            r1[i] += dr;
+        }
+        if (periodic) {
+            const double dr = gamma * (r0[0] - r0[n - 1]);
+            r1[n - 1] -= dr;
+            r1[0] += dr;
+        }
+    } else {
+        const double omega = 2. / 3.;
+        wrk = r0;
+        for (int it = 0; it < 4; ++it) {
+            r1 = r0;
+            for (int i = 0; i < n - 1; ++i) {
+                r1[i] += beta * wrk[i + 1];
+                r1[i + 1] += beta * wrk[i];
+            }
+            for (int i = 1; i < n - 1; ++i) r1[i] *= 1. / (1. + 2. * beta);
+            if (periodic) {
+                r1[0] += beta * wrk[n - 1];
+                r1[n - 1] += beta * wrk[0];
+                r1[0] *= 1. / (1. + 2. * beta);
+                r1[n - 1] *= 1. / (1. + 2. * beta);
+            } else {
+                r1[0] *= 1. / (1. + 1. * beta);
+                r1[n - 1] *= 1. / (1. + 1. * beta);
+            }
+            for (int i = 0; i < n; ++i) wrk[i] = (1. - omega) * wrk[i] + omega * r1[i];
+        }
+        r1 = wrk;
+        for (int i = 0; i < n; ++i) r1[i] *= alpha;
+    }
+    for (int i = 0; i < n; ++i) r[i * stride] = r1[i];
+}
+
+/*
+ * Residual smoothing of a residual field res [N][nvar+1] (the flux part; lambda is left alone),
+ * applied as a product of the 1-D operators: along i on every j-line, then along j on every
+ * i-line; a direction is periodic when its boundaries are.  mode 0 explicit, 1 implicit,
+ * 2 explicit then implicit (the pairing of script/test/testSmooth.cpp:91-95).
+ */
+void orc_smooth_residual(const orc_case* cs, int ni, int nj, int mode, double alpha, double* res)
+{
+    const int nv = nvar_of(cs->law);
+    const bool per_i = cs->bc[0] == ORC_BC_PERIODIC, per_j = cs->bc[2] == ORC_BC_PERIODIC;
+    const int first = mode == 1 ? 1 : 0, last = mode == 0 ? 0 : 1;
+    for (int m = first; m <= last; ++m)
+        for (int v = 0; v < nv; ++v) {
+            for (int j = 0; j < nj; ++j) smooth_line(m, alpha, per_i, ni, res + (size_t)j * (nv + 1) + v, (size_t)nj * (nv + 1));
+            for (int i = 0; i < ni; ++i) smooth_line(m, alpha, per_j, nj, res + (size_t)i * nj * (nv + 1) + v, (size_t)(nv + 1));
+        }
 }
 
 }   // extern "C"

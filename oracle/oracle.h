@@ -99,10 +99,19 @@ int orc_residual(const orc_case* cs, int ni, int nj, const double* nodes, const 
 double orc_spectral_radius(int ni, int nj, const double* cells, const double* res, int nvar);
 int orc_integrate(const orc_case* cs, int ni, int nj, const double* nodes, double* q,
                   const double* bref, int nsteps, double* t_out, double* timers_ms);
+/* integrate with the accumulated stage residual smoothed (orc_smooth_residual) before each update */
+int orc_integrate_smoothed(const orc_case* cs, int ni, int nj, const double* nodes, double* q,
+                           const double* bref, int nsteps, int smooth_mode, double smooth_alpha, double* t_out);
 void orc_norms(int ni, int nj, int nvar, const double* cells, const double* res,
                double* l2, double* linf);
 void orc_set_threads(int n);
 int  orc_get_threads(void);
+
+/* legacy gradient (spatial/gradientCalc.h:16-313): dq [N][2][nvar] */
+void orc_gradient(const orc_case* cs, int ni, int nj, const double* q, double* dq);
+/* residual smoothing, product of the 1-D operators of OLD.fieldOperations/central*Smoothing.ipp;
+   mode 0 explicit, 1 implicit, 2 explicit then implicit */
+void orc_smooth_residual(const orc_case* cs, int ni, int nj, int mode, double alpha, double* res);
 
 /* RK tables (ode.h) and the added classical RK4 */
 void orc_rk_table(int which, int* nstages, double* alpha36, double* beta6);

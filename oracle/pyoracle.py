@@ -80,6 +80,9 @@ def lib(fast=False):
         "orc_set_threads": (None, [i]),
         "orc_get_threads": (i, []),
         "orc_rk_table": (None, [i, P, P, P]),
+        "orc_gradient": (None, [P, i, i, P, P]),
+        "orc_smooth_residual": (None, [P, i, i, i, d, P]),
+        "orc_integrate_smoothed": (i, [P, i, i, P, P, P, i, i, d, P]),
     }
     for fn, (res, args) in sig.items():
         f = getattr(L, fn)
@@ -305,6 +308,20 @@ def integrate(cs, nodes, q, nsteps, bref=None, fast=False):
     return qq, t.value, tm
 
 
+def integrate_smoothed(cs, nodes, q, nsteps, mode, alpha, bref=None):
+    """integrate with the accumulated stage residual smoothed before each update; returns (q, t)."""
+    ni, nj = nodes.shape[0] - 1, nodes.shape[1] - 1
+    if bref is None:
+        bref = np.zeros((bref_size(ni, nj), nvar(cs)))
+    qq = np.ascontiguousarray(q, float).copy()
+    t = C.c_double()
+    rc = lib().orc_integrate_smoothed(_ref(cs), ni, nj, _p(np.ascontiguousarray(nodes)), _p(qq),
+                                      _p(np.ascontiguousarray(bref, float)), nsteps, int(mode), float(alpha), C.byref(t))
+    if rc == -1:
+        raise ValueError("invalid case for the oracle integrate")
+    return qq, t.value
+
+
 def spectral_radius(cells, res):
     ni, nj = cells.shape[:2]
     nv = res.shape[2] - 1
@@ -317,6 +334,22 @@ def norms(cells, res):
     l2, linf = np.zeros(nv), np.zeros(nv)
     lib().orc_norms(ni, nj, nv, _p(np.ascontiguousarray(cells)), _p(np.ascontiguousarray(res)), _p(l2), _p(linf))
     return l2, linf
+
+
+def gradient(cs, q):
+    """Legacy gradient (spatial/gradientCalc.h): [ni][nj][2][nvar] of neighbour differences."""
+    ni, nj = q.shape[:2]
+    out = np.zeros((ni, nj, 2, nvar(cs)))
+    lib().orc_gradient(_ref(cs), ni, nj, _p(np.ascontiguousarray(q, float)), _p(out))
+    return out
+
+
+def smooth_residual(cs, res, mode, alpha):
+    """Residual smoothing of res [ni][nj][nvar+1] (flux part): mode 0 explicit, 1 implicit, 2 both."""
+    out = np.array(res, dtype=float, copy=True, order="C")
+    ni, nj = out.shape[:2]
+    lib().orc_smooth_residual(_ref(cs), ni, nj, int(mode), float(alpha), _p(out))
+    return out
 
 
 def set_threads(n, fast=False):

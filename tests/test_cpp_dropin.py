@@ -103,6 +103,11 @@ int main() {
         rungeKuttaAccumulation(policy, ODE::Explicit::ssp34<Real>(), 0, resStage, res);
         eulerForwardUpdateGlobal(policy, mesh.cells, species, dt, res, q.interior, q1.interior);
         eulerForwardUpdateLocal(policy, species, Real(0.5), res, q.interior, q1.interior);
+        const auto grad = gradientCalc(policy, mesh, q);
+        (void)grad;
+        integrate(policy, SteadyTimeControls<Real>{.nTimesteps = 10, .cfl = 0.5, .residual_drop = 1e-3,
+                  .use_residual_smoothing = true, .smoothing_factor = 2.0}, ODE::Explicit::rk4<Real>(), hoflux, bcs,
+                  species, mesh, q);
         integrate(UnsteadyTimeControls<Real>{.nTimesteps = 10, .cfl = 0.5}, ODE::Explicit::ssp34<Real>(), hoflux, bcs,
                   species, mesh, q);
         integrate(policy, UnsteadyTimeControls<Real>{.nTimesteps = 10, .cfl = 0.5}, ODE::Explicit::ssp34<Real>(), hoflux,
