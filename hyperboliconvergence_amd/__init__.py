@@ -36,7 +36,7 @@ C_ABI_SYMBOLS = (
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
     "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing",
 )
-NTILECFG = 4
+NTILECFG = 3
 
 
 class Species(C.Structure):

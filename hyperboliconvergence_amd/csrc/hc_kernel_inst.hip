@@ -22,33 +22,19 @@ using RF = float;
 template <int K> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
 {
     constexpr hc::TileShape S = hc::tile_shapes(HC_PREC == 32, HC_LAW, HC_FLUX)[K];
-    auto kern = hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst, S.gpf, S.per>;
-    int grid = L.ntiles;
-    if constexpr (S.per != 0) {
-        // resident work-groups of the whole chip, taken once per process (one GPU)
-        static const int resident = [&] {
-            int per_cu = 0, dev = 0, cus = 0;
-            if (hipGetDevice(&dev) != hipSuccess) return 0;
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, S.nt, 0) != hipSuccess) return 0;
-            return per_cu * cus;
-        }();
-        if (resident <= 0) return hipErrorInvalidValue;
-        if (resident < grid) grid = resident;
-    }
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(S.nt), 0, st, L);
+    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst, S.gpf>), dim3(L.ntiles),
+                       dim3(S.nt), 0, st, L);
     return hipGetLastError();
 }
 
 hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::LaunchArgs<RF>& L, int cfg, hipStream_t st)
 {
-    static_assert(hc::NTILECFG == 4, "one launch_shape per tile configuration");
+    static_assert(hc::NTILECFG == 3, "one launch_shape per tile configuration");
     if (L.ntiles <= 0) return hipSuccess;
     switch (cfg) {
     case 0: return launch_shape<0>(L, st);
     case 1: return launch_shape<1>(L, st);
     case 2: return launch_shape<2>(L, st);
-    case 3: return launch_shape<3>(L, st);
     default: return hipErrorInvalidValue;
     }
 }

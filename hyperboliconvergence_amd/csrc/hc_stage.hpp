@@ -823,21 +823,16 @@ static_assert(MAXSPREAD == 64, "one slot per lane of a wavefront");
 // The prefetching configurations hold the prefetched values across the metric phase: at least 4
 // waves per SIMD keeps them within 128 VGPRs (the AUSM+up FP64 kernel would otherwise take 129
 // and drop to 3 waves)
-// PER = 1: persistent launch, the grid holds as many work-groups as are resident at once and each
-// walks the tile table (b, b + grid, ...); with a grid that is a multiple of 8 a work-group keeps
-// its XCD's band of tiles.  PER = 0: one work-group per tile.
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF, int PER = 0>
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF>
 __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GPF != 0 ? 4 : 1)))
 k_stage(LaunchArgs<RF> L)
 {
     using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST, GPF>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
-    const int stride = PER ? (int)gridDim.x : L.ntiles;
-    for (int b = blockIdx.x; b < L.ntiles; b += stride) {
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
     // 8 XCDs, so give each XCD a contiguous band of tiles (shared halos stay in one L2)
-    const int t = L.tile_base + xcd_tile(b, L.ntiles);
+    const int t = L.tile_base + xcd_tile(blockIdx.x, L.ntiles);
     const int ci = L.tile_case[t];
     const CaseRec<RF>& c = L.cases[ci];
     const StageArgs<RF> a = case_args<RF, T::NV>(L, c, ci);
@@ -888,8 +883,6 @@ k_stage(LaunchArgs<RF> L)
     if (a.dt_mode == 2 && a.mode == MODE_FUSED && a.stage == 0 && lt == 0 && tid == 0) {
         a.dtslot[0] = a.cfl;
         a.dtslot[1] = a.dtslot[1] + a.cfl;
-    }
-    if constexpr (PER != 0) __syncthreads();   // the next tile reuses the LDS
     }
 }
 

@@ -10,8 +10,6 @@ struct TileShape {
     int ti, tj, nt;   // cells along i and j, lanes of the work-group
     int cst;          // centre staging: centre differences formed from cell centres in LDS
     int gpf;          // geometry prefetch: each lane's first face slots loaded before the metric phase
-    int per;          // persistent launch: as many work-groups as fit on the chip at once, each
-                      // looping over the tile table (tile b, b + grid, ...)
 };
 
 // Three configurations per precision; a context times each on its own cases and keeps the
@@ -30,13 +28,9 @@ struct TileShape {
 // in FP64 (mixed_wide), so it has the FP64 LDS budget and shapes; the all-FP32 faces of SLAU and
 // AUSM+up take larger tiles: 8x32 (552 faces), 7x32 (487 faces, two per lane) on 256 lanes, or
 // 7x16.
-// The fourth configuration is the third one launched persistently: one launch per stage whose
-// work-groups stay resident and walk the per-case tile table of all cases of the batch.
-constexpr int NTILECFG = 4;
-constexpr TileShape TILES64[NTILECFG] = {
-    {4, 32, 256, 0, 0, 0}, {7, 16, 256, 0, 1, 0}, {7, 16, 256, 1, 1, 0}, {7, 16, 256, 1, 1, 1}};
-constexpr TileShape TILES32[NTILECFG] = {
-    {8, 32, 256, 0, 0, 0}, {7, 32, 256, 0, 0, 0}, {7, 16, 256, 0, 0, 0}, {7, 16, 256, 0, 0, 1}};
+constexpr int NTILECFG = 3;
+constexpr TileShape TILES64[NTILECFG] = {{4, 32, 256, 0, 0}, {7, 16, 256, 0, 1}, {7, 16, 256, 1, 1}};
+constexpr TileShape TILES32[NTILECFG] = {{8, 32, 256, 0, 0}, {7, 32, 256, 0, 0}, {7, 16, 256, 0, 0}};
 
 constexpr const TileShape* tile_shapes(bool mixed, int law, int flux)
 {

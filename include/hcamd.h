@@ -207,15 +207,13 @@ int hc_batch_norms(hc_context* ctx, int case_id, double* l2, double* linf);
 int hc_batch_get_time(hc_context* ctx, int case_id, double* t);
 
 /*
- * Tile configuration of the stage kernels (shape, whether the centre differences are formed from
- * cell centres in LDS, and whether the launch is persistent -- configuration 3: as many work-groups
- * as are resident on the chip, each walking the per-case tile table of all cases): config -1
- * (default) times every configuration on the first steps of the context (all give bit-identical
- * results) and keeps the fastest; 0 .. 3 fixes one.
+ * Tile configuration of the stage kernels (shape, and whether the centre differences are
+ * formed from cell centres in LDS): config -1 (default) times every configuration on the first
+ * steps of the context (all give bit-identical results) and keeps the fastest; 0 .. 2 fixes one.
  * hc_get_tiles returns the configuration in use and, after timing, the milliseconds per step of
  * each (HC_NTILECFG values).  No counterpart in the reference.
  */
-#define HC_NTILECFG 4
+#define HC_NTILECFG 3
 int hc_set_tiles(hc_context* ctx, int config);
 int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step);
 

@@ -170,10 +170,8 @@ template <int K> void stage_shape(Emu& e, StageArgs<double>& a64, StageArgs<floa
 
 void stage(Emu& e, StageArgs<double>& a64, StageArgs<float>& a32)
 {
-    static_assert(NTILECFG == 4, "one stage_shape per tile configuration");
-    // the persistent configuration runs configuration 2's tiles (the host walks every tile anyway)
-    if (g_tile_cfg == 3) stage_shape<3>(e, a64, a32);
-    else if (g_tile_cfg == 2) stage_shape<2>(e, a64, a32);
+    static_assert(NTILECFG == 3, "one stage_shape per tile configuration");
+    if (g_tile_cfg == 2) stage_shape<2>(e, a64, a32);
     else if (g_tile_cfg == 1) stage_shape<1>(e, a64, a32);
     else stage_shape<0>(e, a64, a32);
 }
