@@ -34,7 +34,7 @@ C_ABI_SYMBOLS = (
     "hc_set_tiles", "hc_get_tiles",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
-    "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing",
+    "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing", "hc_spectral_radius_noalloc",
 )
 NTILECFG = 3
 
@@ -117,6 +117,7 @@ def library():
         "hc_gradient": (i, [P, P, P]),
         "hc_smooth_residual": (i, [P, i, d, P]),
         "hc_set_residual_smoothing": (i, [P, i, d]),
+        "hc_spectral_radius_noalloc": (i, [P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -358,6 +359,12 @@ class Context:
     def set_residual_smoothing(self, mode, alpha=1.0):
         """Smooth every stage's accumulated residual in the time loop (mode -1 off, 0..2 as smooth_residual)."""
         self._check(self._L.hc_set_residual_smoothing(self._h, int(mode), float(alpha)), "hc_set_residual_smoothing")
+
+    def spectral_radius_noalloc(self, res_dev_ptr, out_dev_ptr):
+        """max lambda / volume of a device residual [ni][nj][nv+1] into a device double, allocating
+        nothing (asynchronous on the context's stream)."""
+        self._check(self._L.hc_spectral_radius_noalloc(self._h, C.c_void_p(int(res_dev_ptr)),
+                                                       C.c_void_p(int(out_dev_ptr))), "hc_spectral_radius_noalloc")
 
     def stream_triad(self, n, reps):
         g = C.c_double()

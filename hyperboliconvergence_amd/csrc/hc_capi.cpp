@@ -781,7 +781,7 @@ int hc_create(hc_context** out, int device)
         const unsigned int zs = 0;
         e = hipMemcpy(ctx->status, &zs, sizeof(zs), hipMemcpyHostToDevice);
     }
-    if (e != hipSuccess) {
+    if (e != hipSuccess || noalloc_setup(device) != HC_OK) {
         hc_destroy(ctx);
         return HC_ERR_HIP;
     }

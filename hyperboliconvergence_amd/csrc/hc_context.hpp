@@ -117,6 +117,7 @@ template <class RF> hc::LaunchArgs<RF> launch_args(hc_context* ctx);
 void release_adjoint(hc_context* ctx);
 int smooth_device(hc_context* ctx, int mode, double alpha, double* r, double* b, double* c, int stride);
 int enqueue_step_smoothed(hc_context* ctx);
+int noalloc_setup(int device);
 void drop_graph(hc_context* ctx);
 
 }   // namespace hc_detail

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <mutex>
 #include <vector>
 
 namespace {
@@ -136,6 +137,13 @@ __global__ void __launch_bounds__(256) k_specrad_final(const double* part, int n
     }
     if (threadIdx.x == 0) out[0] = sm[0];
 }
+
+// Scratch of the allocation-free spectral radius (hc_spectral_radius_noalloc): the partial maxima
+// of the first reduction pass (8 KiB); one copy per GPU, held until the process ends.  Host code
+// never passes the array's name to a kernel: it takes the device address once per device
+// (noalloc_setup) and passes that.
+constexpr int NOALLOC_PARTS = 1024;
+static __device__ __align__(256) double scratch[NOALLOC_PARTS];
 
 // q1 = S( C(q0) + R (coef / vol) ) with mode 0 (eulerForwardUpdateGlobal, eulerForwardUpdate.h:38-92,
 // coef = dt), or q1 = S( C(q0) + R (coef / lambda) ) with mode 1 (eulerForwardUpdateLocal, :117-168,
@@ -406,6 +414,22 @@ int staging(hc_context* ctx, size_t ndoubles, double** out)
 
 
 namespace hc_detail {
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static double* g_scratch[NOALLOC_MAX_DEVICES] = {};
+static std::mutex g_scratch_mu;
+
+// the device address of `scratch` on the current device (device), taken once per device
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return HC_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch[device]) return HC_OK;
+    void* ptr = nullptr;
+    if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return HC_ERR_HIP;
+    g_scratch[device] = static_cast<double*>(ptr);
+    return HC_OK;
+}
 
 // Residual smoothing of the flux part of r [N][stride] on the device (mode 0 explicit, 1 implicit,
 // 2 explicit then implicit; each operator along i, then along j); b and c are scratch buffers of
@@ -695,6 +719,31 @@ int hc_set_residual_smoothing(hc_context* ctx, int mode, double alpha)
     drop_graph(ctx);
     ctx->smooth_mode = mode;
     ctx->smooth_alpha = mode >= 0 ? alpha : 1.0;
+    return HC_OK;
+}
+
+// The spectral radius of a residual already on the device, without any allocation: the partial
+// maxima live in the file-scope scratch of the GPU (every call overwrites the slots it reads, so
+// nothing needs clearing); the result is written to the device address out_dev.  Asynchronous on
+// the context's stream (hc_synchronize waits).  Calls that overlap on one GPU share its scratch
+// and race.
+int hc_spectral_radius_noalloc(hc_context* ctx, const double* res_dev, double* out_dev)
+{
+    int r = single_case(ctx);
+    if (r) return r;
+    if (!res_dev || !out_dev) return fail(ctx, HC_ERR_ARG, "null device pointer");
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return fail(ctx, HC_ERR_ARG, "device index out of range");
+    double* part = g_scratch[ctx->device];
+    if (!part) return fail(ctx, HC_ERR_STATE, "scratch address of the device not set up (hc_create)");
+    HC_HIP(hipSetDevice(ctx->device));
+    const long long N = ctx->cases[0].N;
+    const long long nb = (N + 255) / 256;
+    const int nparts = (int)(nb < NOALLOC_PARTS ? nb : NOALLOC_PARTS);   // grid-stride: any N fits
+    if (nparts < 1 || nparts > NOALLOC_PARTS) return fail(ctx, HC_ERR_ARG, "partial count beyond the scratch");
+    hipLaunchKernelGGL(k_specrad_partials, dim3(nparts), dim3(256), 0, ctx->stream, res_dev, ctx->cvol, N, ctx->nv,
+                       part);
+    hipLaunchKernelGGL(k_specrad_final, dim3(1), dim3(256), 0, ctx->stream, part, nparts, out_dev);
+    HC_HIP(hipGetLastError());
     return HC_OK;
 }
 

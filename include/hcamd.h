@@ -295,6 +295,18 @@ int hc_smooth_residual(hc_context* ctx, int mode, double alpha, double* res);
  */
 int hc_set_residual_smoothing(hc_context* ctx, int mode, double alpha);
 
+/*
+ * Allocation-free variant of hc_spectral_radius for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid allocation); it has no
+ * counterpart in the reference.  res_dev: a device array [N][nvar+1] of the context's (single)
+ * case; out_dev: device address of one double that receives max over cells of lambda / volume.
+ * The reduction's partial maxima live in a fixed per-GPU scratch of 1024 doubles that the library
+ * holds until the process ends (any N fits: the first pass strides over the cells with at most
+ * 1024 work-groups); the call allocates nothing and is asynchronous on the context's stream
+ * (hc_synchronize waits).  Calls that overlap on one GPU share that scratch and race.
+ */
+int hc_spectral_radius_noalloc(hc_context* ctx, const double* res_dev, double* out_dev);
+
 /* Library version string. */
 const char* hc_version(void);
 
