@@ -98,12 +98,13 @@ def first_nonfinite_step(cfg, steps, chunk=50):
     return None
 
 
-def run_gpu(cfg, precision, steps, warmup, graph_steps):
+def run_gpu(cfg, precision, steps, warmup, graph_steps, uniform=1):
     import hyperboliconvergence_amd as hca
     ctx = hca.Context(0)
     ctx.set_mesh(cfg["nodes"])
     ctx.set_case(cfg["case"])
     ctx.set_precision(precision)
+    ctx.set_uniform_geometry(uniform)
     ctx.set_boundary_refs(cfg["bref"])
     ctx.set_state(cfg["q"])
     ctx.set_graph_steps(graph_steps)
@@ -124,6 +125,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     region = ctx.profile()
     ctx.set_profiling(0)
     tiles = ctx.tiles()
+    uni = ctx.uniform_geometry()
     q = ctx.get_state()
     tsim = ctx.time()
     st = ctx.status()
@@ -135,7 +137,7 @@ def run_gpu(cfg, precision, steps, warmup, graph_steps):
     per_launch = ctx.profile()
     ctx.set_profiling(0)
     ctx.close()
-    return t1 - t0, q, st, (region, per_launch, tiles, tsim), warmup
+    return t1 - t0, q, st, (region, per_launch, tiles, tsim, uni), warmup
 
 
 def cpu_baseline(cfg, budget_s=12.0):
@@ -164,14 +166,14 @@ def cpu_baseline(cfg, budget_s=12.0):
                       f"steady_clock around the time loop"}
 
 
-def traffic_from_profiles(workload, precision, tile_cfg):
-    """HBM bytes per stage-kernel launch of the tile configuration in use, measured with rocprofv3
-    FETCH_SIZE/WRITE_SIZE (profiles/traffic.json)."""
+def traffic_from_profiles(workload, precision, tile_cfg, uniform=False):
+    """HBM bytes per stage-kernel launch of the tile configuration in use (and of the uniform-geometry
+    kernels when they ran), measured with rocprofv3 FETCH_SIZE/WRITE_SIZE (profiles/traffic.json)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             d = json.load(f)
-        return d.get(f"{workload}:{'mixed' if precision else 'fp64'}:{tile_cfg}")
+        return d.get(f"{workload}:{'mixed' if precision else 'fp64'}:{tile_cfg}" + (":uni" if uniform else ""))
     except (OSError, ValueError):
         return None
 
@@ -478,6 +480,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the divergence search and the face-frame Roe runs (for counter collection)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--uniform-geometry", type=int, default=1, choices=[0, 1],
+                    help="1: uniform-geometry stage kernels where the mesh allows them (c1, c3); 0: general kernels")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -503,14 +507,15 @@ def main():
 
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
-        el, q, st, (region, per_launch, tiles, tsim), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps)
+        el, q, st, (region, per_launch, tiles, tsim, uni), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps,
+                                                                            a.uniform_geometry)
         # stage-kernel time: the events' interval over the timed steps per stage launch (it also
         # holds the global-dt update kernels and the gaps between launches: an upper bound)
         kern_ms = region[0] / max(region[1], 1.0)
         res[prec] = dict(seconds=el, q=q, status=st, kernel_ms=kern_ms,
                          launch_ms=per_launch[0] / max(per_launch[1], 1.0),
                          upd_ms=per_launch[2] / max(per_launch[3], 1.0), rate=N * ns * steps / el,
-                         tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]], tsim=tsim)
+                         tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]], tsim=tsim, uni=bool(uni))
         print(f"# {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, "
               f"{res[prec]['rate']:.4e} cell-stage/s, stage kernel {kern_ms * 1e3:.2f} us, status {st}, "
               f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})",
@@ -557,7 +562,7 @@ def main():
         ff = {}
         ff_steps = cfg["nsteps"]
         for prec in (0, 1):
-            el, q, st, (region, _, _, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
+            el, q, st, (region, _, _, _, _), _ = run_gpu(cfg_ff, prec, ff_steps, a.warmup, a.graph_steps)
             ff[prec] = dict(rate=N * ns * ff_steps / el, q=q, status=st, kernel_us=region[0] / max(region[1], 1.0) * 1e3)
         ff_err = rel_err_conserved(ff[1]["q"], ff[0]["q"], cs.law, cs.basis)
         print(f"# face-frame Roe: fp64 {ff[0]['rate']:.4e}, mixed {ff[1]['rate']:.4e} cell-stage/s, "
@@ -573,7 +578,8 @@ def main():
     alg_bytes = bytes_per_cell_stage * N
     achieved = alg_bytes / (h["kernel_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(a.workload, head, h["tiles"]),
+                "frac": achieved / HBM_PEAK_GBS,
+                "traffic": traffic_from_profiles(a.workload, head, h["tiles"], h["uni"]),
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bytes_per_cell_stage,
                 "kernel_us": h["kernel_ms"] * 1e3, "kernel_us_eager_events": h["launch_ms"] * 1e3,
                 "update_kernel_us": h["upd_ms"] * 1e3 if cs.dt_mode == 0 else None}
@@ -600,7 +606,7 @@ def main():
         "config": {"workload": cfg["name"], "cells": N, "ni": ni, "nj": nj, "rk_stages": ns,
                    "precision": "mixed" if head == 1 else "fp64", "graph_steps": a.graph_steps,
                    "configured_steps": cfg["nsteps"], "tile_shape": h["tiles"],
-                   "tile_shape_ms_per_step": h["tile_ms"]},
+                   "tile_shape_ms_per_step": h["tile_ms"], "uniform_geometry": h["uni"]},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "fp64_value": res[0]["rate"] if 0 in res else None,

@@ -31,7 +31,7 @@ C_ABI_SYMBOLS = (
     "hc_get_profile", "hc_device_state", "hc_stream_triad", "hc_version",
     "hc_batch_begin", "hc_batch_add_case", "hc_batch_end", "hc_batch_size", "hc_batch_set_boundary_refs",
     "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
-    "hc_set_tiles", "hc_get_tiles",
+    "hc_set_tiles", "hc_get_tiles", "hc_set_uniform_geometry", "hc_get_uniform_geometry",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
     "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing", "hc_spectral_radius_noalloc",
@@ -104,6 +104,8 @@ def library():
         "hc_batch_get_time": (i, [P, i, P]),
         "hc_set_tiles": (i, [P, i]),
         "hc_get_tiles": (i, [P, P, P]),
+        "hc_set_uniform_geometry": (i, [P, i]),
+        "hc_get_uniform_geometry": (i, [P, P]),
         "hc_tangent_residual": (i, [P, P, P, d, P, P]),
         "hc_adjoint_residual": (i, [P, P, P, P, P]),
         "hc_adjoint_forward": (i, [P, i, i]),
@@ -249,6 +251,16 @@ class Context:
         ms = np.zeros(NTILECFG)
         self._check(self._L.hc_get_tiles(self._h, C.byref(cfg), _p(ms)), "hc_get_tiles")
         return cfg.value, ms
+
+    def set_uniform_geometry(self, enable):
+        """1 (default): the uniform-geometry stage kernels when every case allows them; 0: never."""
+        self._check(self._L.hc_set_uniform_geometry(self._h, int(enable)), "hc_set_uniform_geometry")
+
+    def uniform_geometry(self):
+        """Whether the uniform-geometry stage kernels are in use."""
+        act = C.c_int()
+        self._check(self._L.hc_get_uniform_geometry(self._h, C.byref(act)), "hc_get_uniform_geometry")
+        return bool(act.value)
 
     # ---- derivatives (single case, FP64) ----
     def _brows(self):

@@ -355,6 +355,8 @@ int build_layout(hc_context* ctx)
         HC_HIP(hipMemcpy(ctx->fi32 + NSLOT * c.fi_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
         fp32_slots(c.g.fj, c.nfj, f32);
         HC_HIP(hipMemcpy(ctx->fj32 + NSLOT * c.fj_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
+        const GeomInput gin{c.ni, c.nj, nullptr, {c.cs.bc[0], c.cs.bc[1], c.cs.bc[2], c.cs.bc[3]}, {0.0, 0.0}, {0.0, 0.0}};
+        c.uniform = uniform_geometry(gin, c.g);
         c.X = std::move(c.g.X);
         c.g = GeomOutput{};   // the device holds it now
     }
@@ -404,6 +406,8 @@ int build_layout(hc_context* ctx)
     HC_HIP(hipMalloc(&ctx->chunks, ch.size() * sizeof(UpdChunk)));
     HC_HIP(hipMemcpy(ctx->chunks, ch.data(), ch.size() * sizeof(UpdChunk), hipMemcpyHostToDevice));
     ctx->nchunks = (int)ch.size();
+    ctx->uni_ok = true;
+    for (const auto& c : ctx->cases) ctx->uni_ok = ctx->uni_ok && c.uniform;
     ctx->have_case = true;
     return HC_OK;
 }
@@ -447,10 +451,11 @@ hipError_t launch_stage(hc_context* ctx, const LaunchArgs<double>& L)
 {
     const int key = ctx->cs.law * 100 + ctx->cs.basis * 10 + ctx->cs.flux;
     const int cfg = ctx->tile_cfg;
+    const int uni = ctx->uni_ok && ctx->uni_enable ? 1 : 0;
     if (ctx->precision == HC_PRECISION_MIXED) {
         const LaunchArgs<float> L32 = with_cases<float>(L, ctx->rec32[cfg]);
 #define HC_CASE_STAGE32(Lw, B, F) \
-    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P32(L32, cfg, ctx->stream);
+    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P32(L32, cfg, uni, ctx->stream);
         switch (key) {
             HC_FOR_EACH_KERNEL(HC_CASE_STAGE32)
         default: return hipErrorInvalidValue;
@@ -458,7 +463,7 @@ hipError_t launch_stage(hc_context* ctx, const LaunchArgs<double>& L)
 #undef HC_CASE_STAGE32
     }
 #define HC_CASE_STAGE64(Lw, B, F) \
-    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P64(L, cfg, ctx->stream);
+    case Lw * 100 + B * 10 + F: return hc_stage_L##Lw##_B##B##_F##F##_P64(L, cfg, uni, ctx->stream);
     switch (key) {
         HC_FOR_EACH_KERNEL(HC_CASE_STAGE64)
     default: return hipErrorInvalidValue;
@@ -1074,6 +1079,25 @@ int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step)
     *config = ctx->tile_cfg;
     if (ms_per_step)
         for (int k = 0; k < NTILECFG; ++k) ms_per_step[k] = ctx->tuned ? ctx->tune_ms[k] : 0.0;
+    return HC_OK;
+}
+
+int hc_set_uniform_geometry(hc_context* ctx, int enable)
+{
+    if (!ctx) return HC_ERR_ARG;
+    if (enable != 0 && enable != 1) return fail(ctx, HC_ERR_ARG, "enable must be 0 or 1");
+    if (ctx->uni_enable != (enable != 0)) {
+        drop_graph(ctx);       // captured launches name the other kernels
+        ctx->tuned = false;    // and their timings differ
+    }
+    ctx->uni_enable = enable != 0;
+    return HC_OK;
+}
+
+int hc_get_uniform_geometry(hc_context* ctx, int* active)
+{
+    if (!ctx || !active) return HC_ERR_ARG;
+    *active = ctx->have_case && ctx->uni_ok && ctx->uni_enable ? 1 : 0;
     return HC_OK;
 }
 

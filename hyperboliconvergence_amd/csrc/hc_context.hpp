@@ -32,6 +32,7 @@ struct CaseHost {
     long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
     int brows = 0;                 // boundary reference rows 2 nj + 2 ni
     bool have_state = false;
+    bool uniform = false;          // uniform geometry (hc_geom.hpp uniform_geometry)
     int tiles_j[NTILECFG] = {}, tile0[NTILECFG] = {}, ntiles[NTILECFG] = {};
 };
 
@@ -74,6 +75,8 @@ struct hc_context {
     // tile shape: fixed index, or -1 to time every shape on the first steps and keep the fastest
     int tile_mode = -1;
     int tile_cfg = 0;
+    bool uni_ok = false;           // every case has uniform geometry
+    bool uni_enable = true;        // use the uniform-geometry kernels when uni_ok (hc_set_uniform_geometry)
     bool tuned = false;
     double tune_ms[NTILECFG] = {};
     // graphs

@@ -5,6 +5,7 @@
 #include "hc_stage.hpp"
 
 #include <cmath>
+#include <cstring>
 
 namespace hc {
 namespace {
@@ -213,6 +214,28 @@ int build_geometry(const GeomInput& in, GeomOutput& out)
             }
         }
     return 0;
+}
+
+bool uniform_geometry(const GeomInput& in, const GeomOutput& g)
+{
+    const int ni = in.ni, nj = in.nj;
+    if (in.bc[0] != BC_PERIODIC || in.bc[2] != BC_PERIODIC || ni < 1 || nj < 1) return false;
+    auto same = [](double a, double b) { return std::memcmp(&a, &b, sizeof(double)) == 0; };
+    const long long N = (long long)ni * nj;
+    for (long long c = 1; c < N; ++c)
+        if (!same(g.cvol[c], g.cvol[0])) return false;
+    const long long nfi = (long long)(ni + 1) * nj, nfj = (long long)ni * (nj + 1);
+    for (int s = 0; s < NSLOT; ++s) {
+        const double* fi = g.fi.data() + s * nfi;
+        const double* fj = g.fj.data() + s * nfj;
+        for (int l = 0; l < ni; ++l)
+            for (int t = 0; t < nj; ++t)
+                if (!same(fi[(long long)l * nj + t], fi[0])) return false;
+        for (int i = 0; i < ni; ++i)
+            for (int l = 0; l < nj; ++l)
+                if (!same(fj[(long long)i * (nj + 1) + l], fj[0])) return false;
+    }
+    return true;
 }
 
 void fp32_slots(const std::vector<double>& f64, long long nf, std::vector<float>& f32)

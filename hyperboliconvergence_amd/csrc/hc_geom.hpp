@@ -29,6 +29,13 @@ struct GeomOutput {
 // returns 0 on success, -1 on a degenerate cell or face
 int build_geometry(const GeomInput& in, GeomOutput& out);
 
+// Uniform geometry: the mesh is periodic in both directions and every face slot of the faces the
+// stage kernel reads (i-faces at node lines 0 .. ni-1, j-faces at node lines 0 .. nj-1) is bitwise
+// equal to that of face 0 of its kind, and every cell volume to that of cell 0 (a uniform Cartesian
+// grid of a periodic box).  The stage kernel then takes the slots of face 0 and the volume of cell 0
+// for all faces and cells, with the same bits as the per-face arrays.
+bool uniform_geometry(const GeomInput& in, const GeomOutput& g);
+
 // FP32 face slots of the mixed path from the FP64 ones: every slot rounded, except that the two
 // biased LSQ factors become their directional weights w = X'^-1 n (two per side: the slope along
 // n is Q'.w), formed in FP64 and rounded

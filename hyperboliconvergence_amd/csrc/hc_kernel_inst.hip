@@ -19,22 +19,33 @@ using RF = double;
 using RF = float;
 #endif
 
-template <int K> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
+// UNI: the uniform-geometry kernel of the shape; centre staging has nothing to stage there, so a
+// shape that differs from another only by it shares that shape's kernel
+template <int K, int UNI> static hipError_t launch_shape(const hc::LaunchArgs<RF>& L, hipStream_t st)
 {
     constexpr hc::TileShape S = hc::tile_shapes(HC_PREC == 32, HC_LAW, HC_FLUX)[K];
-    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, S.cst, S.gpf>), dim3(L.ntiles),
+    constexpr int CST = UNI ? 0 : S.cst;
+    hipLaunchKernelGGL((hc::k_stage<HC_LAW, HC_BASIS, HC_FLUX, RF, S.ti, S.tj, S.nt, CST, S.gpf, UNI>), dim3(L.ntiles),
                        dim3(S.nt), 0, st, L);
     return hipGetLastError();
 }
 
-hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::LaunchArgs<RF>& L, int cfg, hipStream_t st)
+hipError_t HC_NAME(HC_LAW, HC_BASIS, HC_FLUX, HC_PREC)(const hc::LaunchArgs<RF>& L, int cfg, int uni, hipStream_t st)
 {
     static_assert(hc::NTILECFG == 3, "one launch_shape per tile configuration");
     if (L.ntiles <= 0) return hipSuccess;
+    if (uni) {
+        switch (cfg) {
+        case 0: return launch_shape<0, 1>(L, st);
+        case 1: return launch_shape<1, 1>(L, st);
+        case 2: return launch_shape<2, 1>(L, st);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (cfg) {
-    case 0: return launch_shape<0>(L, st);
-    case 1: return launch_shape<1>(L, st);
-    case 2: return launch_shape<2>(L, st);
+    case 0: return launch_shape<0, 0>(L, st);
+    case 1: return launch_shape<1, 0>(L, st);
+    case 2: return launch_shape<2, 0>(L, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -14,8 +14,8 @@
 #define HC_FOR_EACH_LAWBASIS(X) X(0, 0) X(1, 0) X(1, 1)
 
 #define HC_DECL_STAGE(L, B, F)                                                                             \
-    hipError_t hc_stage_L##L##_B##B##_F##F##_P64(const hc::LaunchArgs<double>&, int cfg, hipStream_t);     \
-    hipError_t hc_stage_L##L##_B##B##_F##F##_P32(const hc::LaunchArgs<float>&, int cfg, hipStream_t);
+    hipError_t hc_stage_L##L##_B##B##_F##F##_P64(const hc::LaunchArgs<double>&, int cfg, int uni, hipStream_t); \
+    hipError_t hc_stage_L##L##_B##B##_F##F##_P32(const hc::LaunchArgs<float>&, int cfg, int uni, hipStream_t);
 #define HC_DECL_UPD(L, B)                                                                                  \
     hipError_t hc_upd_L##L##_B##B##_P64(const hc::LaunchArgs<double>&, const hc::UpdChunk*, int, hipStream_t); \
     hipError_t hc_upd_L##L##_B##B##_P32(const hc::LaunchArgs<float>&, const hc::UpdChunk*, int, hipStream_t);

@@ -168,8 +168,12 @@ HC_HD int xcd_tile(int b, int ntiles)
 }
 
 // NTHR threads per work-group, at least one per cell: the extra lanes share the load, metric and
-// face phases, so more waves are resident per SIMD for the same LDS tile
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ, int CST = 0, int GPF = 0>
+// face phases, so more waves are resident per SIMD for the same LDS tile.
+// UNI: uniform geometry (a periodic box whose face slots and cell volumes are all equal, checked
+// bitwise on the host): every face takes the slots of face 0 of its kind and every cell the volume
+// of cell 0, read at wave-uniform addresses (scalar loads) instead of one load per face and slot.
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR = TI * TJ, int CST = 0, int GPF = 0,
+          int UNI = 0>
 struct StageTile {
     static constexpr int NV = LawTraits<LAW>::nv;
     static constexpr int NC = TI * TJ;   // cells of the tile
@@ -193,7 +197,7 @@ struct StageTile {
     // the centre differences dx_c of the metric edges and faces are formed there, with the
     // host geometry's operations (c_R - c_L, and c_R - (c_L - period) across a periodic wrap),
     // instead of being read as two face slots per edge.
-    static constexpr bool CSTAGE = CST != 0 && (sizeof(RF) == sizeof(double) || MIXED);
+    static constexpr bool CSTAGE = CST != 0 && UNI == 0 && (sizeof(RF) == sizeof(double) || MIXED);
     static constexpr int CH = CSTAGE ? QH : 1, CW = CSTAGE ? QW : 1;
 
     struct Smem {
@@ -274,7 +278,10 @@ struct StageTile {
             }
             auto edge = [&](int rl, int cl, int rr, int cr, const RF* base, long long nf, long long fidx, bool wrap) {
                 RF dx0, dx1;
-                if constexpr (CSTAGE) {
+                if constexpr (UNI != 0) {
+                    dx0 = base[FS_DX0 * nf];
+                    dx1 = base[FS_DX1 * nf];
+                } else if constexpr (CSTAGE) {
                     double d0, d1;
                     centre_diff(a, s, rl * QW + cl, rr * QW + cr, wrap, cl == cr, d0, d1);
                     dx0 = RF(d0);
@@ -345,6 +352,28 @@ struct StageTile {
                 g.fl[d] = base[(FS_FL0 + d) * nf + fidx];
                 g.fr[d] = base[(FS_FR0 + d) * nf + fidx];
             }
+        }
+        return g;
+    }
+
+    // uniform geometry: the slots of i-face 0 and j-face 0, each read at one wave-uniform address,
+    // and the lane's kind selected
+    HC_HD static Geo uni_geo(const StageArgs<RF>& a, bool isi)
+    {
+        const Geo gi = load_geo(a.m.fi, a.m.fi64, a.m.nfi, 0, 0.0, 0.0);
+        const Geo gj = load_geo(a.m.fj, a.m.fj64, a.m.nfj, 0, 0.0, 0.0);
+        Geo g;
+        g.area = isi ? gi.area : gj.area;
+        g.n0 = isi ? gi.n0 : gj.n0;
+        g.n1 = isi ? gi.n1 : gj.n1;
+        g.dl = isi ? gi.dl : gj.dl;
+        g.dr = isi ? gi.dr : gj.dr;
+        g.s1 = isi ? gi.s1 : gj.s1;
+        g.dx0 = isi ? gi.dx0 : gj.dx0;
+        g.dx1 = isi ? gi.dx1 : gj.dx1;
+        for (int d = 0; d < 3; ++d) {
+            g.fl[d] = isi ? gi.fl[d] : gj.fl[d];
+            g.fr[d] = isi ? gi.fr[d] : gj.fr[d];
         }
         return g;
     }
@@ -455,7 +484,7 @@ struct StageTile {
     // difference of centre staging is filled in the face phase
     HC_HD static void prefetch(const StageArgs<RF>& a, int i0, int j0, int tid, Geo& g)
     {
-        if constexpr (GPF != 0) {
+        if constexpr (GPF != 0 && UNI == 0) {
             const int k = tid;
             if (k >= NF) return;
             const bool isi = k < NFI;
@@ -513,7 +542,9 @@ struct StageTile {
                 }
                 double cx0 = 0.0, cx1 = 0.0;
                 if constexpr (CSTAGE) centre_diff(a, s, pl, pr, !interior, isi, cx0, cx1);
-                if (GPF != 0 && k == tid) {
+                if constexpr (UNI != 0) {
+                    store(s, k, muscl(a, uni_geo(a, isi), ql, qr, Ql, Qr));
+                } else if (GPF != 0 && k == tid) {
                     Geo g = pre;
                     if constexpr (CSTAGE) {
                         g.dx0 = RF(cx0);
@@ -568,7 +599,7 @@ struct StageTile {
             const int fs = first_acc_slot(a.plan);
             if (fs >= 0)
                 for (int v = 0; v < NV; ++v) ci.acc[v] = a.acc[((long long)fs * NV + v) * a.N + c];
-            ci.vol = a.m.cvol[c];
+            ci.vol = a.m.cvol[UNI ? 0 : c];
             if (a.dt_mode == 1 && a.stage != 0) ci.lam0 = a.lam0[c];
         }
     }
@@ -639,7 +670,7 @@ struct StageTile {
                 *A = old + R[v] * p.upd_coef[u];
             }
         }
-        const double vol = CPF ? ci.vol : a.m.cvol[c];
+        const double vol = CPF ? ci.vol : a.m.cvol[UNI ? 0 : c];
         if (a.mode == MODE_GLOBAL_STAGE0) {
             for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
             red = lam / vol;   // spectralRadius.h:33-35
@@ -823,11 +854,11 @@ static_assert(MAXSPREAD == 64, "one slot per lane of a wavefront");
 // The prefetching configurations hold the prefetched values across the metric phase: at least 4
 // waves per SIMD keeps them within 128 VGPRs (the AUSM+up FP64 kernel would otherwise take 129
 // and drop to 3 waves)
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF>
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NTHR, int CST, int GPF, int UNI = 0>
 __global__ void __launch_bounds__(NTHR) HC_STAGE_OCCUPANCY __attribute__((amdgpu_waves_per_eu(GPF != 0 ? 4 : 1)))
 k_stage(LaunchArgs<RF> L)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST, GPF>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NTHR, CST, GPF, UNI>;
     __shared__ typename T::Smem s;
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the

@@ -218,6 +218,17 @@ int hc_set_tiles(hc_context* ctx, int config);
 int hc_get_tiles(hc_context* ctx, int* config, double* ms_per_step);
 
 /*
+ * Uniform geometry: when every case of the context is a periodic box whose face geometry and cell
+ * volumes are bitwise the same for all faces of a kind and all cells (a uniform Cartesian grid),
+ * the stage kernels read one set of face slots per kind and one volume per case instead of one
+ * per face and cell; results are bit-identical.  enable = 1 (default) uses those kernels whenever
+ * the mesh allows it, 0 always uses the general ones.  hc_get_uniform_geometry reports whether the
+ * uniform kernels are in use.  No counterpart in the reference.
+ */
+int hc_set_uniform_geometry(hc_context* ctx, int enable);
+int hc_get_uniform_geometry(hc_context* ctx, int* active);
+
+/*
  * Derivatives (single-case context, FP64; Euler with every flux in the conserved basis, AUSM+up
  * also in the primitive basis).  No counterpart in the reference.
  * Parameters: the state q, minf of the species (AUSM+up / SLAU / RoeUnprecWS scaling) and the boundary

@@ -2,8 +2,8 @@
 // from a file, for a build under AddressSanitizer and UndefinedBehaviorSanitizer (every load and
 // store of the kernels' index arithmetic checked against the buffers it addresses).
 //
-// Input (native byte order): int32[8] {ni, nj, precision, nsteps, tile configuration, adjoint,
-// nv, sizeof(hc_case_desc)}, the hc_case_desc bytes, then doubles: nodes [ni+1][nj+1][2],
+// Input (native byte order): int32[8] {ni, nj, precision, nsteps, tile configuration (+16: the
+// uniform-geometry kernels where the case allows them), adjoint, nv, sizeof(hc_case_desc)}, the hc_case_desc bytes, then doubles: nodes [ni+1][nj+1][2],
 // q [ni][nj][nv], bref [2nj+2ni][nv] and, when adjoint != 0, qd [ni][nj][nv] and ybar [ni][nj][nv].
 // Output, doubles: residual [ni][nj][nv+1], the state after nsteps [ni][nj][nv], the time and the
 // status; when adjoint != 0 also the tangent residual [ni][nj][nv+1] and its derivative
@@ -16,6 +16,7 @@
 
 extern "C" {
 int emu_set_tile_config(int cfg);
+int emu_set_uniform_geometry(int enable);
 int emu_residual(const hc_case_desc* cs, int ni, int nj, const double* nodes, const double* q, const double* bref,
                  int precision, double* res, unsigned int* status);
 int emu_integrate(const hc_case_desc* cs, int ni, int nj, const double* nodes, double* q, const double* bref,
@@ -64,7 +65,8 @@ int main(int argc, char** argv)
         fprintf(stderr, "asan_driver: short input\n");
         return 2;
     }
-    if (emu_set_tile_config(tiles) != 0) return 3;
+    if (emu_set_tile_config(tiles & 15) != 0) return 3;
+    emu_set_uniform_geometry(tiles >> 4);
 
     std::vector<double> res(N * (nv + 1)), qs(q), tail(2);
     unsigned int st = 0;

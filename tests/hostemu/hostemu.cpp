@@ -79,10 +79,13 @@ template <class RF> StageArgs<RF> args_of(Emu& e)
     return a;
 }
 
-template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST, int GPF>
-void run_stage(const StageArgs<RF>& a)
+// uniform-geometry kernels (UNI) when the case allows them, selected with emu_set_uniform_geometry
+int g_uniform = 0;
+
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST, int GPF, int UNI>
+void run_stage_t(const StageArgs<RF>& a)
 {
-    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST, GPF>;
+    using T = StageTile<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST, GPF, UNI>;
     const int tiles_i = (a.m.ni + TI - 1) / TI, tiles_j = (a.m.nj + TJ - 1) / TJ;
     const int ntiles = tiles_i * tiles_j;
     auto sp = std::make_unique<typename T::Smem>();
@@ -113,6 +116,15 @@ void run_stage(const StageArgs<RF>& a)
         *a.status |= f;
     }
     if (a.reset_max) *a.maxslot = dbl_bits(DBL_MIN);
+}
+
+bool g_case_uniform = false;
+
+template <int LAW, int BASIS, int FLUX, class RF, int TI, int TJ, int NT, int CST, int GPF>
+void run_stage(const StageArgs<RF>& a)
+{
+    if (g_uniform && g_case_uniform) run_stage_t<LAW, BASIS, FLUX, RF, TI, TJ, NT, 0, GPF, 1>(a);
+    else run_stage_t<LAW, BASIS, FLUX, RF, TI, TJ, NT, CST, GPF, 0>(a);
 }
 
 template <int LAW, int BASIS, class RF> void run_update(const StageArgs<RF>& a)
@@ -193,6 +205,7 @@ int setup(Emu& e, const hc_case_desc* cs, int ni, int nj, const double* nodes, c
     GeomInput gin{ni, nj, nodes, {cs->bc[0], cs->bc[1], cs->bc[2], cs->bc[3]}, {cs->period_i[0], cs->period_i[1]},
                   {cs->period_j[0], cs->period_j[1]}};
     if (build_geometry(gin, e.g) != 0) return -1;
+    g_case_uniform = uniform_geometry(gin, e.g);
     fp32_slots(e.g.fi, (long long)(ni + 1) * nj, e.fi32);
     fp32_slots(e.g.fj, (long long)ni * (nj + 1), e.fj32);
     const long long N = (long long)ni * nj;
@@ -212,6 +225,14 @@ int setup(Emu& e, const hc_case_desc* cs, int ni, int nj, const double* nodes, c
 }   // namespace
 
 extern "C" {
+
+// 1: run the uniform-geometry kernels on cases whose geometry allows them; returns whether the
+// last set-up case allowed them
+int emu_set_uniform_geometry(int enable)
+{
+    g_uniform = enable;
+    return g_case_uniform ? 1 : 0;
+}
 
 int emu_set_tile_config(int cfg)
 {

@@ -20,6 +20,7 @@ def lib():
         _L.emu_integrate.argtypes = [P, i, i, P, P, P, i, i, P, P]
         _L.emu_geometry.argtypes = [P, i, i, P, P, P, P, P]
         _L.emu_set_tile_config.argtypes = [i]
+        _L.emu_set_uniform_geometry.argtypes = [i]
         _L.emu_tangent_residual.argtypes = [P, i, i, P, P, P, P, P, C.c_double, P, P]
         _L.emu_adjoint_residual.argtypes = [P, i, i, P, P, P, P, P, P, P]
     return _L
@@ -28,6 +29,12 @@ def lib():
 def set_tile_config(cfg):
     """Tile shape of the emulated launches (index into the kernel's shape table)."""
     assert lib().emu_set_tile_config(int(cfg)) == 0
+
+
+def set_uniform_geometry(enable):
+    """1: emulate the uniform-geometry kernels on cases that allow them; returns whether the last
+    case set up allowed them."""
+    return bool(lib().emu_set_uniform_geometry(int(enable)))
 
 
 def _p(a):

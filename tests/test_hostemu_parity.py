@@ -130,3 +130,51 @@ def test_adaptive_low_mach_scaling_bit_identical(flux):
     qe, _, _ = pyemu.integrate(cs, nodes, q, bref, 5)
     qo, _, _ = po.integrate(cs, nodes, q, 5, bref)
     assert np.array_equal(qe, qo)
+
+
+@pytest.mark.parametrize("flux", range(7))
+@pytest.mark.parametrize("precision", [0, 1], ids=["fp64", "mixed"])
+def test_uniform_geometry_kernels_bit_identical_to_the_general_ones(flux, precision):
+    """On a periodic box whose spacing is exact in binary (h = 20/40) the host finds the geometry
+    uniform; the uniform-geometry kernels (one slot set per face kind, one volume) then give the
+    general kernels' bits, and the oracle's in FP64, over 4 SSP-RK3 steps with a global time step."""
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_isentropic_vortex(40, flux)
+    out = {}
+    for uni in (1, 0):
+        pyemu.set_uniform_geometry(uni)
+        try:
+            out[uni] = pyemu.integrate(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"], 4, precision=precision)
+            allowed = pyemu.set_uniform_geometry(0)
+        finally:
+            pyemu.set_uniform_geometry(0)
+        assert allowed
+    assert np.array_equal(out[1][0], out[0][0], equal_nan=True) and out[1][1] == out[0][1]
+    if precision == 0:
+        qo, to, _ = po.integrate(cfg["case"], cfg["nodes"], cfg["q"], 4, cfg["bref"])
+        assert np.array_equal(out[1][0], qo, equal_nan=True) and out[1][1] == to
+
+
+def test_uniform_geometry_is_found_only_where_every_slot_is_equal():
+    """A box whose spacing is not exact in binary (h = 20/24: volumes differ in the last bit), a
+    periodic strip with a wall, and the cylinder O-grid are not uniform; the kernels then stay the
+    general ones."""
+    from hyperboliconvergence_amd import cases
+    import hyperboliconvergence_amd as hca
+    pyemu.set_uniform_geometry(1)
+    try:
+        cfg = cases.config_isentropic_vortex(24, hca.FLUX_AUSMPUP)
+        pyemu.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"])
+        assert not pyemu.set_uniform_geometry(1)
+        cfg = cases.config_isentropic_vortex(32, hca.FLUX_AUSMPUP)
+        pyemu.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"])
+        assert pyemu.set_uniform_geometry(1)
+        cfg["case"].bc[2] = po.BC_WALL
+        cfg["case"].bc[3] = po.BC_WALL
+        pyemu.residual(cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"])
+        assert not pyemu.set_uniform_geometry(1)
+        cs, nodes, q, bref = _ogrid(8, 16, 1, po.FLUX_ROE, po.LIM_MINMOD2)
+        pyemu.residual(cs, nodes, q, bref)
+        assert not pyemu.set_uniform_geometry(1)
+    finally:
+        pyemu.set_uniform_geometry(0)

@@ -58,15 +58,18 @@ def _run(driver, tmp_path, cs, nodes, q, bref, precision=0, nsteps=3, tiles=0, a
     return parts
 
 
-def _check_forward(driver, tmp_path, cfg, precision, tiles, nsteps=3):
+def _check_forward(driver, tmp_path, cfg, precision, tiles, nsteps=3, uniform=0):
     cs, nodes, q, bref = cfg["case"], cfg["nodes"], cfg["q"], cfg["bref"]
-    res, qs, tail = _run(driver, tmp_path, cs, nodes, q, bref, precision, nsteps, tiles)
+    res, qs, tail = _run(driver, tmp_path, cs, nodes, q, bref, precision, nsteps, tiles + 16 * uniform)
     pyemu.set_tile_config(tiles)
+    pyemu.set_uniform_geometry(uniform)
     try:
         r_ref, _ = pyemu.residual(cs, nodes, q, bref, precision)
         q_ref, t_ref, st_ref = pyemu.integrate(cs, nodes, q, bref, nsteps, precision)
+        assert pyemu.set_uniform_geometry(0) == bool(uniform)
     finally:
         pyemu.set_tile_config(0)
+        pyemu.set_uniform_geometry(0)
     assert np.array_equal(res, r_ref.ravel(), equal_nan=True)
     assert np.array_equal(qs, q_ref.ravel(), equal_nan=True)
     assert tail[0] == t_ref and tail[1] == st_ref
@@ -92,6 +95,17 @@ def test_forward_kernels_in_bounds_and_bit_identical(driver, tmp_path, name, mak
             if name.startswith("scalar") and precision == 1:
                 continue    # the reference VanLeer2 scalar case grows without bound in FP32 faces
             _check_forward(driver, tmp_path, cfg, precision, tiles)
+
+
+@pytest.mark.parametrize("flux", [1, 2, 4])
+def test_uniform_geometry_kernels_in_bounds_and_bit_identical(driver, tmp_path, flux):
+    """The uniform-geometry kernels on a 40 x 40 periodic box (h = 0.5, ragged against every tile
+    shape): in bounds, and the ordinary build's bits."""
+    from hyperboliconvergence_amd import cases
+    cfg = cases.config_isentropic_vortex(40, flux)
+    for tiles in range(NTILE):
+        for precision in (0, 1):
+            _check_forward(driver, tmp_path, cfg, precision, tiles, uniform=1)
 
 
 @pytest.mark.parametrize("flux,limiter", SMOOTH)
