@@ -436,3 +436,26 @@ def test_implicit_residual_smoothing_accelerates_the_steady_march():
     q_plain, _, _ = po.integrate(cs, cfg["nodes"], cfg["q"], 200, cfg["bref"])
     q_smooth, _ = po.integrate_smoothed(cs, cfg["nodes"], cfg["q"], 200, 1, 2.0, cfg["bref"])
     assert drop(q_smooth) < 1e-2 < drop(q_plain)
+
+
+# ---------------- reductions ----------------
+def test_spectral_radius_and_norms_closed_forms():
+    """spectralRadius (spatial/spectralRadius.h:19-48): max over cells of lambda / volume, from the
+    smallest normal number; the residual norms: per variable sqrt(mean((R / V)^2)) and max |R / V|.
+    Checked against numpy on a random residual, and on a residual whose spectral column is zero."""
+    rng = np.random.default_rng(3)
+    ni, nj, nv = 7, 9, 4
+    cells = np.zeros((ni, nj, 3))
+    cells[..., 0] = rng.uniform(0.5, 2.0, (ni, nj))
+    cells[..., 1:] = rng.standard_normal((ni, nj, 2))
+    res = rng.standard_normal((ni, nj, nv + 1))
+    res[..., nv] = np.abs(res[..., nv])
+    vol = cells[..., 0]
+    assert po.spectral_radius(cells, res) == np.max(res[..., nv] / vol)
+    z = res.copy()
+    z[..., nv] = 0.0
+    assert po.spectral_radius(cells, z) == np.finfo(np.float64).tiny
+    l2, linf = po.norms(cells, res)
+    r = res[..., :nv] / vol[..., None]
+    assert np.allclose(l2, np.sqrt(np.mean(r * r, axis=(0, 1))), rtol=1e-14, atol=0)
+    assert np.array_equal(linf, np.max(np.abs(r), axis=(0, 1)))
