@@ -156,6 +156,36 @@ def test_global_time_step_every_rk_table_matches_oracle(rk):
         assert abs(t - to) <= 1e-12 * abs(to), graph_steps
 
 
+def test_graph_recapture_after_an_odd_remainder_and_a_tile_change_equals_eager():
+    """Forward Euler (one stage: the state buffers alternate, so a graph is recaptured whenever a
+    step count leaves the other buffer current): 101 steps in graphs of 50, then 7 more, a change
+    of tile configuration, then 53 more; each recapture destroys the previous graph while its last
+    launches may still be in flight.  The state equals eager stepping and the oracle."""
+    from hyperboliconvergence_amd import cases
+    cfg = _vortex_cfg(64, 4)
+    n, a, b = cases.rk_table("ssp11")
+    cs = cfg["case"]
+    cs.nstages = n
+    for s in range(6):
+        cs.beta[s] = b[s]
+        for k in range(6):
+            cs.alpha[s][k] = a[s, k]
+    out = {}
+    for graph_steps in (0, 50):
+        with _ctx(cfg) as ctx:
+            ctx.set_tiles(0)
+            ctx.set_graph_steps(graph_steps)
+            ctx.step(101)
+            ctx.step(7)
+            ctx.set_tiles(1)
+            ctx.step(53)
+            out[graph_steps] = (ctx.get_state(), ctx.time())
+            assert ctx.status() == 0
+    assert np.array_equal(out[0][0], out[50][0]) and out[0][1] == out[50][1]
+    qo, to, _ = po.integrate(cs, cfg["nodes"], cfg["q"], 161, cfg["bref"])
+    assert rel_err(out[50][0], qo) <= TRAJ_TOL
+
+
 def test_gresho_reference_driver_matches_oracle():
     """script/euler/gresho2D.cpp's settings: 256^2, Rusanov + NoLimit3, SSP(3,4), global dt at
     CFL 1.6, 10 steps, M = 0.3."""

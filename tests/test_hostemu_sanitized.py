@@ -75,11 +75,28 @@ def _check_forward(driver, tmp_path, cfg, precision, tiles, nsteps=3, uniform=0)
     assert tail[0] == t_ref and tail[1] == st_ref
 
 
+def _entropy_bc(cfg):
+    cfg["case"].bc[1] = po.BC_ENTROPY
+    return cfg
+
+
+def _conserved(cfg):
+    from hyperboliconvergence_amd import cases
+    cfg["case"].basis = po.BASIS_CONSERVED
+    cfg["q"], cfg["bref"] = cases.prim_to_cons(cfg["q"]), cases.prim_to_cons(cfg["bref"])
+    return cfg
+
+
 def _cases():
     from hyperboliconvergence_amd import cases
     out = []
     for flux in range(7):
         out.append((f"cylinder-20x12-flux{flux}", lambda f=flux: cases.config_cylinder(20, 12, 0.5, 10.0, flux=f)))
+    out.append(("cylinder-20x12-entropy-bc", lambda: _entropy_bc(cases.config_cylinder(20, 12, 0.5, 10.0))))
+    out.append(("cylinder-20x12-conserved-roe", lambda: _conserved(cases.config_cylinder(20, 12, 0.5, 10.0,
+                                                                                         flux=cases.FLUX_ROE))))
+    out.append(("cylinder-20x12-conserved-ausmpup",
+                lambda: _conserved(cases.config_cylinder(20, 12, 0.5, 10.0, flux=cases.FLUX_AUSMPUP))))
     out.append(("scalar-10x10", lambda: cases.config_scalar_advection(10)))
     out.append(("vortex-12x12-ausmpup", lambda: cases.config_isentropic_vortex(12, cases.FLUX_AUSMPUP)))
     out.append(("shocktube-9x4-roe", lambda: cases.config_shocktube(9, cases.FLUX_ROE)))
