@@ -33,6 +33,7 @@ C_ABI_SYMBOLS = (
     "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
     "hc_set_tiles", "hc_get_tiles", "hc_set_uniform_geometry", "hc_get_uniform_geometry",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
+    "hc_set_state_device", "hc_get_state_device", "hc_set_boundary_refs_device", "hc_adjoint_backward_device",
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
     "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing", "hc_spectral_radius_noalloc",
 )
@@ -67,6 +68,13 @@ def library():
         return _LIB
     if not os.path.exists(LIBPATH):
         raise HCError(f"{LIBPATH} is missing: build it with `python -m hyperboliconvergence_amd.build`")
+    # One HIP runtime per process: torch's wheel carries its own libamdhip64.so.7.  Loaded first, it
+    # also serves this library (same soname), so device pointers of torch tensors are valid here;
+    # loaded after this library, it would be a second runtime in which torch finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIBPATH)
     P, i, d = C.c_void_p, C.c_int, C.c_double
     sig = {
@@ -110,6 +118,10 @@ def library():
         "hc_adjoint_residual": (i, [P, P, P, P, P]),
         "hc_adjoint_forward": (i, [P, i, i]),
         "hc_adjoint_backward": (i, [P, P, P, P, P]),
+        "hc_set_state_device": (i, [P, P]),
+        "hc_get_state_device": (i, [P, P]),
+        "hc_set_boundary_refs_device": (i, [P, P]),
+        "hc_adjoint_backward_device": (i, [P, P, P, P, P]),
         "hc_tangent_steps": (i, [P, i, P, P, d, P]),
         "hc_xmetrics": (i, [P, P]),
         "hc_qmetrics": (i, [P, P, P]),
@@ -291,6 +303,26 @@ class Context:
         self._check(self._L.hc_adjoint_residual(self._h, _p(ybar), _p(qbar), _p(bb), C.byref(mb)),
                     "hc_adjoint_residual")
         return qbar, bb, mb.value
+
+    # ---- device arrays (integer device addresses, e.g. torch.Tensor.data_ptr() of float64 tensors) ----
+    def set_state_device(self, q_ptr):
+        self._check(self._L.hc_set_state_device(self._h, C.c_void_p(int(q_ptr))), "hc_set_state_device")
+
+    def get_state_device(self, q_ptr):
+        self._check(self._L.hc_get_state_device(self._h, C.c_void_p(int(q_ptr))), "hc_get_state_device")
+
+    def set_boundary_refs_device(self, bref_ptr):
+        self._check(self._L.hc_set_boundary_refs_device(self._h, C.c_void_p(int(bref_ptr))),
+                    "hc_set_boundary_refs_device")
+
+    def adjoint_backward_device(self, qbar_final_ptr, qbar_initial_ptr, bref_bar_ptr):
+        """Device arrays in, device arrays out; returns the minf adjoint."""
+        mb = C.c_double()
+        self._check(self._L.hc_adjoint_backward_device(self._h, C.c_void_p(int(qbar_final_ptr)),
+                                                       C.c_void_p(int(qbar_initial_ptr)),
+                                                       C.c_void_p(int(bref_bar_ptr)), C.byref(mb)),
+                    "hc_adjoint_backward_device")
+        return mb.value
 
     def adjoint_forward(self, nsteps, checkpoint_every):
         self._check(self._L.hc_adjoint_forward(self._h, int(nsteps), int(checkpoint_every)), "hc_adjoint_forward")

@@ -384,13 +384,16 @@ int hc_adjoint_forward(hc_context* ctx, int nsteps, int checkpoint_every)
     return check_status(ctx);
 }
 
-int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_initial, double* bref_bar,
-                        double* minf_bar)
+// dev: qbar_final, qbar_initial and bref_bar are device arrays (minf_bar stays a host scalar)
+static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_initial, double* bref_bar,
+                            double* minf_bar, bool dev)
 {
     int r = adj_ready(ctx, true);
     if (r) return r;
     if (!ctx->adj || !ctx->adj->have_traj) return fail(ctx, HC_ERR_STATE, "hc_adjoint_forward must come first");
     if (!qbar_final || !qbar_initial || !bref_bar || !minf_bar) return fail(ctx, HC_ERR_ARG, "null argument");
+    if (dev && !(device_pointer(qbar_final) && device_pointer(qbar_initial) && device_pointer(bref_bar)))
+        return fail(ctx, HC_ERR_ARG, "qbar_final, qbar_initial and bref_bar must be device memory of the library's HIP runtime");
     AdjState& s = *ctx->adj;
     const auto& c = ctx->cases[0];
     const int ns = s.ns, nv = s.nv;
@@ -400,7 +403,12 @@ int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_
         (r = grow(ctx, s.yb, ns, len)))
         return r;
     // qbn holds the adjoint of the state at the current step boundary
-    if ((r = upload_soa(ctx, s.qbn, qbar_final, N, nv))) return r;
+    if (dev) {
+        HC_HIP(hipStreamSynchronize(ctx->stream));
+        if ((r = aos_to_soa_device(ctx, s.qbn, qbar_final, N, nv))) return r;
+    } else if ((r = upload_soa(ctx, s.qbn, qbar_final, N, nv))) {
+        return r;
+    }
     HC_HIP(hipMemsetAsync(s.brefbar_tot, 0, nb * sizeof(double), ctx->stream));
     HC_HIP(hipMemsetAsync(s.scal, 0, 4 * sizeof(double), ctx->stream));
     const int nseg = (s.K + s.C - 1) / s.C;
@@ -448,10 +456,28 @@ int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_
             }
         }
     }
-    if ((r = download_soa(ctx, qbar_initial, s.qbn, N, nv))) return r;
-    HC_HIP(hipMemcpy(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToHost));
+    if (dev) {
+        if ((r = soa_to_aos_device(ctx, qbar_initial, s.qbn, N, nv))) return r;
+        HC_HIP(hipMemcpyAsync(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        HC_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        if ((r = download_soa(ctx, qbar_initial, s.qbn, N, nv))) return r;
+        HC_HIP(hipMemcpy(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToHost));
+    }
     HC_HIP(hipMemcpy(minf_bar, s.scal, sizeof(double), hipMemcpyDeviceToHost));
     return check_status(ctx);
+}
+
+int hc_adjoint_backward(hc_context* ctx, const double* qbar_final, double* qbar_initial, double* bref_bar,
+                        double* minf_bar)
+{
+    return adjoint_backward(ctx, qbar_final, qbar_initial, bref_bar, minf_bar, false);
+}
+
+int hc_adjoint_backward_device(hc_context* ctx, const double* qbar_final_dev, double* qbar_initial_dev,
+                               double* bref_bar_dev, double* minf_bar)
+{
+    return adjoint_backward(ctx, qbar_final_dev, qbar_initial_dev, bref_bar_dev, minf_bar, true);
 }
 
 int hc_tangent_steps(hc_context* ctx, int nsteps, const double* qdot0, const double* bref_dot, double minf_dot,

@@ -7,6 +7,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -128,6 +129,21 @@ __global__ void __launch_bounds__(256) k_fill(double* a, double v, long long n)
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) a[i] = v;
 }
 
+// [N][nv] (cell-major, as the C-ABI's host arrays) <-> [nv][N] planes, both in device memory
+__global__ void __launch_bounds__(256) k_aos_to_soa(const double* __restrict__ aos, long long N, int nv,
+                                                    double* __restrict__ soa)
+{
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += (long long)gridDim.x * 256)
+        for (int v = 0; v < nv; ++v) soa[v * N + c] = aos[c * nv + v];
+}
+
+__global__ void __launch_bounds__(256) k_soa_to_aos(const double* __restrict__ soa, long long N, int nv,
+                                                    double* __restrict__ aos)
+{
+    for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < N; c += (long long)gridDim.x * 256)
+        for (int v = 0; v < nv; ++v) aos[c * nv + v] = soa[v * N + c];
+}
+
 template <class T> void free_dev(T*& p)
 {
     if (p) (void)hipFree(p);
@@ -138,6 +154,37 @@ template <class T> void free_dev(T*& p)
 
 
 namespace hc_detail {
+
+// whether p is device (or managed) memory of this HIP runtime; a failed query's error is cleared
+// so that it is not reported by a later hipGetLastError
+bool device_pointer(const void* p)
+{
+    if (!p) return false;
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return at.type == hipMemoryTypeDevice || at.type == hipMemoryTypeManaged || at.isManaged;
+}
+
+int aos_to_soa_device(hc_context* ctx, double* soa, const double* aos_dev, long long N, int nv)
+{
+    const long long nb = std::min<long long>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((unsigned)std::max<long long>(nb, 1)), dim3(256), 0, ctx->stream, aos_dev, N,
+                       nv, soa);
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
+
+int soa_to_aos_device(hc_context* ctx, double* aos_dev, const double* soa, long long N, int nv)
+{
+    const long long nb = std::min<long long>((N + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((unsigned)std::max<long long>(nb, 1)), dim3(256), 0, ctx->stream, soa, N,
+                       nv, aos_dev);
+    HC_HIP(hipGetLastError());
+    return HC_OK;
+}
 
 using namespace hc;
 
@@ -951,6 +998,46 @@ int hc_set_precision(hc_context* ctx, int precision)
 }
 
 int hc_set_boundary_refs(hc_context* ctx, const double* bref) { return set_boundary_refs(ctx, 0, bref); }
+
+int hc_set_state_device(hc_context* ctx, const double* q_dev)
+{
+    int r = case_ready(ctx, 0);
+    if (r) return r;
+    if (!device_pointer(q_dev)) return fail(ctx, HC_ERR_ARG, "q_dev is not device memory of the library's HIP runtime");
+    CaseHost& c = ctx->cases[0];
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    if ((r = aos_to_soa_device(ctx, ctx->q[ctx->cur] + ctx->nv * c.cell_off, q_dev, c.N, ctx->nv))) return r;
+    const double z[2] = {0.0, 0.0};
+    HC_HIP(hipMemcpyAsync(ctx->dtslot, z, sizeof(z), hipMemcpyHostToDevice, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    c.have_state = true;
+    return HC_OK;
+}
+
+int hc_get_state_device(hc_context* ctx, double* q_dev)
+{
+    int r = case_ready(ctx, 0);
+    if (r) return r;
+    const CaseHost& c = ctx->cases[0];
+    if (!c.have_state) return fail(ctx, HC_ERR_STATE, "no state");
+    if (!device_pointer(q_dev)) return fail(ctx, HC_ERR_ARG, "q_dev is not device memory of the library's HIP runtime");
+    if ((r = soa_to_aos_device(ctx, q_dev, ctx->q[ctx->cur] + ctx->nv * c.cell_off, c.N, ctx->nv))) return r;
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
+
+int hc_set_boundary_refs_device(hc_context* ctx, const double* bref_dev)
+{
+    int r = case_ready(ctx, 0);
+    if (r) return r;
+    if (!device_pointer(bref_dev))
+        return fail(ctx, HC_ERR_ARG, "bref_dev is not device memory of the library's HIP runtime");
+    const CaseHost& c = ctx->cases[0];
+    HC_HIP(hipMemcpyAsync(ctx->bref + ctx->nv * c.bref_off, bref_dev, (size_t)c.brows * ctx->nv * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    return HC_OK;
+}
 int hc_set_state(hc_context* ctx, const double* q) { return set_state(ctx, 0, q); }
 int hc_get_state(hc_context* ctx, double* q) { return get_state(ctx, 0, q); }
 int hc_residual(hc_context* ctx, double* res) { return residual(ctx, 0, res); }

@@ -121,6 +121,9 @@ void release_adjoint(hc_context* ctx);
 int smooth_device(hc_context* ctx, int mode, double alpha, double* r, double* b, double* c, int stride);
 int enqueue_step_smoothed(hc_context* ctx);
 int noalloc_setup(int device);
+bool device_pointer(const void* p);
+int aos_to_soa_device(hc_context* ctx, double* soa, const double* aos_dev, long long N, int nv);
+int soa_to_aos_device(hc_context* ctx, double* aos_dev, const double* soa, long long N, int nv);
 void drop_graph(hc_context* ctx);
 
 }   // namespace hc_detail

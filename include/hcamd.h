@@ -256,6 +256,22 @@ int hc_tangent_steps(hc_context* ctx, int nsteps, const double* qdot0, const dou
                      double* qdot_final);
 
 /*
+ * Device-array variants for callers whose data is already in device memory of the HIP runtime the
+ * library uses (for instance PyTorch tensors on the GPU, in a process where torch's HIP runtime is
+ * the one loaded): the same layouts and meaning as hc_set_state, hc_get_state,
+ * hc_set_boundary_refs (solutionField/solutionField.h:10-140, the SolutionField's interior and
+ * boundary arrays) and hc_adjoint_backward, for the context's single case; minf_bar stays a host
+ * scalar.  The caller's writes to the inputs must be complete before the call (its stream
+ * synchronised); the calls return once the outputs are written.  A pointer that is not device
+ * memory of that runtime is rejected with HC_ERR_ARG.
+ */
+int hc_set_state_device(hc_context* ctx, const double* q_dev);
+int hc_get_state_device(hc_context* ctx, double* q_dev);
+int hc_set_boundary_refs_device(hc_context* ctx, const double* bref_dev);
+int hc_adjoint_backward_device(hc_context* ctx, const double* qbar_final_dev, double* qbar_initial_dev,
+                               double* bref_bar_dev, double* minf_bar);
+
+/*
  * Spatial building blocks of a single-case context, for callers that run the time loop
  * themselves on host arrays (the reference's integrate() body, timestepping/rungeKutta.h:97-164);
  * hc_step runs the same arithmetic fused.  Arrays as above; the geometry, boundary types, basis

@@ -204,3 +204,40 @@ def test_backward_after_a_second_forward_on_the_same_solver_raises():
         assert torch.isfinite(q0.grad).all()
     finally:
         solver.close()
+
+
+def test_cuda_tensors_stay_on_the_gpu_and_give_the_host_path_bits():
+    """The device-array entry points: torch tensors on the GPU (the library shares torch's HIP
+    runtime) give the same forward state and gradients as CPU tensors through the host-array entry
+    points; set/get state round-trips on the device; a host pointer is refused, not dereferenced."""
+    import torch
+    import hyperboliconvergence_amd as hca
+    from hyperboliconvergence_amd.adjoint import TrajectorySolver, trajectory
+    hca.library()
+    assert torch.cuda.is_available(), "torch and libhcamd must share one HIP runtime"
+    cfg = _cfg(nsteps=8)
+    out = {}
+    for dev in ("cpu", "cuda"):
+        q0 = torch.tensor(cfg["q"], dtype=torch.float64, device=dev, requires_grad=True)
+        bref = torch.tensor(cfg["bref"], dtype=torch.float64, device=dev, requires_grad=True)
+        M = torch.tensor(cfg["case"].sp.minf, dtype=torch.float64, requires_grad=True)
+        solver = TrajectorySolver(cfg["nodes"], cfg["case"], 8, 3)
+        try:
+            qK = trajectory(solver, q0, bref, M)
+            assert qK.device.type == dev
+            (qK * qK).sum().backward()
+        finally:
+            solver.close()
+        out[dev] = [t.detach().cpu().numpy() for t in (qK, q0.grad, bref.grad, M.grad)]
+    for a, b in zip(out["cpu"], out["cuda"]):
+        assert np.array_equal(a, b)
+    with _ctx(cfg) as ctx:
+        q = torch.tensor(cfg["q"] * 1.01, dtype=torch.float64, device="cuda")
+        ctx.set_state_device(q.data_ptr())
+        assert np.array_equal(ctx.get_state(), cfg["q"] * 1.01)
+        back = torch.empty_like(q)
+        ctx.get_state_device(back.data_ptr())
+        assert torch.equal(back, q)
+        host = np.ascontiguousarray(cfg["q"])
+        with pytest.raises(hca.HCError, match="device memory"):
+            ctx.set_state_device(host.ctypes.data)

@@ -1,7 +1,7 @@
 """GPU: the allocation-free spectral radius (hc_spectral_radius_noalloc) against the allocating
 hc_spectral_radius through the C-ABI, on two calls in a row on one GPU, the second on a smaller
 case (the scratch keeps the first call's partials).  The device arrays are taken with the HIP
-runtime the library itself links (/opt/rocm/lib/libamdhip64.so)."""
+runtime the library is bound to (looked up by its soname, so the one already loaded)."""
 import ctypes as C
 
 import numpy as np
@@ -11,7 +11,9 @@ pytestmark = pytest.mark.gpu
 
 
 def _hip():
-    L = C.CDLL("/opt/rocm/lib/libamdhip64.so")
+    import hyperboliconvergence_amd as hca
+    hca.library()
+    L = C.CDLL("libamdhip64.so.7")
     L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
     L.hipFree.argtypes = [C.c_void_p]
     L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
