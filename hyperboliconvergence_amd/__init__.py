@@ -34,6 +34,7 @@ C_ABI_SYMBOLS = (
     "hc_set_tiles", "hc_get_tiles", "hc_set_uniform_geometry", "hc_get_uniform_geometry",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
     "hc_set_state_device", "hc_get_state_device", "hc_set_boundary_refs_device", "hc_adjoint_backward_device",
+    "hc_step_norms",
     "hc_xmetrics", "hc_qmetrics", "hc_boundary_update", "hc_spectral_radius", "hc_forward_update",
     "hc_gradient", "hc_smooth_residual", "hc_set_residual_smoothing", "hc_spectral_radius_noalloc",
 )
@@ -119,6 +120,7 @@ def library():
         "hc_adjoint_forward": (i, [P, i, i]),
         "hc_adjoint_backward": (i, [P, P, P, P, P]),
         "hc_set_state_device": (i, [P, P]),
+        "hc_step_norms": (i, [P, i, P, P]),
         "hc_get_state_device": (i, [P, P]),
         "hc_set_boundary_refs_device": (i, [P, P]),
         "hc_adjoint_backward_device": (i, [P, P, P, P, P]),
@@ -232,6 +234,12 @@ class Context:
     def norms(self):
         l2, linf = np.zeros(self.nv), np.zeros(self.nv)
         self._check(self._L.hc_norms(self._h, _p(l2), _p(linf)), "hc_norms")
+        return l2, linf
+
+    def step_norms(self, nsteps):
+        """nsteps steps; the norms of the residual of the last step's start state (fused into it)."""
+        l2, linf = np.zeros(self.nv), np.zeros(self.nv)
+        self._check(self._L.hc_step_norms(self._h, int(nsteps), _p(l2), _p(linf)), "hc_step_norms")
         return l2, linf
 
     def status(self):

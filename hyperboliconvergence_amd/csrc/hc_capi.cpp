@@ -33,45 +33,58 @@ __global__ void k_clear_dt(double* p, int n)
 
 __global__ void k_clear_u32(unsigned int* p) { *p = 0u; }
 
-// partial sums of (R/vol)^2 and max |R/vol| per variable; one partial per work-group
-__global__ void __launch_bounds__(256) k_norm_partials(const double* res, const double* vol, long long N, int nv,
-                                                       double* part)
+// Residual norms with wavefront reductions: each lane folds its cells (grid stride), each
+// wavefront folds its 64 lanes with cross-lane shuffles (six exchanges, no LDS traffic), and one
+// LDS exchange combines the four wavefronts of a work-group.  The order of every sum is fixed by
+// the launch shape, so results are reproducible run to run.
+__device__ inline double wave_sum_d(double v)
 {
-    __shared__ double ss[4][256];
-    __shared__ double sm[4][256];
+    for (int off = 32; off > 0; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    return v;
+}
+__device__ inline double wave_max_d(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// one partial per work-group: sums of (R/vol)^2 and max |R/vol| per variable, [8] per partial
+__global__ void __launch_bounds__(256) k_norm_partials(const double* __restrict__ res, const double* __restrict__ vol,
+                                                       long long N, int nv, double* __restrict__ part)
+{
+    __shared__ double w8[4][8];
     const int tid = threadIdx.x;
     double s[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
     for (long long c = (long long)blockIdx.x * 256 + tid; c < N; c += (long long)gridDim.x * 256) {
+        const double vc = vol[c];
         for (int v = 0; v < nv; ++v) {
-            const double r = res[c * (nv + 1) + v] / vol[c];
+            const double r = res[c * (nv + 1) + v] / vc;
             s[v] = s[v] + r * r;
             m[v] = fmax(m[v], fabs(r));
         }
     }
     for (int v = 0; v < 4; ++v) {
-        ss[v][tid] = s[v];
-        sm[v][tid] = m[v];
+        s[v] = wave_sum_d(s[v]);
+        m[v] = wave_max_d(m[v]);
     }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int v = 0; v < 4; ++v) {
-                ss[v][tid] = ss[v][tid] + ss[v][tid + w];
-                sm[v][tid] = fmax(sm[v][tid], sm[v][tid + w]);
-            }
-        __syncthreads();
-    }
-    if (tid == 0)
+    if ((tid & 63) == 0)
         for (int v = 0; v < 4; ++v) {
-            part[blockIdx.x * 8 + v] = ss[v][0];
-            part[blockIdx.x * 8 + 4 + v] = sm[v][0];
+            w8[tid >> 6][v] = s[v];
+            w8[tid >> 6][4 + v] = m[v];
         }
+    __syncthreads();
+    if (tid < 8) {
+        double a = w8[0][tid];
+        for (int w = 1; w < 4; ++w) a = tid < 4 ? a + w8[w][tid] : fmax(a, w8[w][tid]);
+        part[blockIdx.x * 8 + tid] = a;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_norm_final(const double* part, int nparts, long long N, double* out)
+// the case's norms from nparts partials: {L2[4] = sqrt(sum / N), Linf[4]}
+__global__ void __launch_bounds__(256) k_norm_final(const double* __restrict__ part, int nparts, long long N,
+                                                    double* __restrict__ out)
 {
-    __shared__ double ss[4][256];
-    __shared__ double sm[4][256];
+    __shared__ double w8[4][8];
     const int tid = threadIdx.x;
     double s[4] = {0, 0, 0, 0}, m[4] = {0, 0, 0, 0};
     for (int p = tid; p < nparts; p += 256)
@@ -80,23 +93,20 @@ __global__ void __launch_bounds__(256) k_norm_final(const double* part, int npar
             m[v] = fmax(m[v], part[p * 8 + 4 + v]);
         }
     for (int v = 0; v < 4; ++v) {
-        ss[v][tid] = s[v];
-        sm[v][tid] = m[v];
+        s[v] = wave_sum_d(s[v]);
+        m[v] = wave_max_d(m[v]);
     }
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (tid < w)
-            for (int v = 0; v < 4; ++v) {
-                ss[v][tid] = ss[v][tid] + ss[v][tid + w];
-                sm[v][tid] = fmax(sm[v][tid], sm[v][tid + w]);
-            }
-        __syncthreads();
-    }
-    if (tid == 0)
+    if ((tid & 63) == 0)
         for (int v = 0; v < 4; ++v) {
-            out[v] = sqrt(ss[v][0] / (double)N);
-            out[4 + v] = sm[v][0];
+            w8[tid >> 6][v] = s[v];
+            w8[tid >> 6][4 + v] = m[v];
         }
+    __syncthreads();
+    if (tid < 8) {
+        double a = w8[0][tid];
+        for (int w = 1; w < 4; ++w) a = tid < 4 ? a + w8[w][tid] : fmax(a, w8[w][tid]);
+        out[tid] = tid < 4 ? sqrt(a / (double)N) : a;
+    }
 }
 
 constexpr int NORM_BLOCKS = 1024;
@@ -608,6 +618,7 @@ int enqueue_step(hc_context* ctx)
         L.plan = ctx->plans[s];
         L.reset_max = (global && ns >= 2 && s == ns - 1) ? 1 : 0;
         L.mode = (global && s == 0) ? MODE_GLOBAL_STAGE0 : MODE_FUSED;
+        L.keep_res = (ctx->keep_res && s == 0) ? 1 : 0;
         hipError_t e = timed(ctx, 0, [&] { return launch_stage(ctx, L); });
         if (e != hipSuccess) return hipfail(ctx, e, "stage kernel launch");
         if (global && s == 0) {
@@ -732,13 +743,9 @@ int residual(hc_context* ctx, int k, double* res)
     return check_status(ctx);
 }
 
-int norms(hc_context* ctx, int k, double* l2, double* linf)
+// norms of the residual of case k held in ctx->res
+int norms_of_res(hc_context* ctx, int k, double* l2, double* linf)
 {
-    int r = case_ready(ctx, k);
-    if (r) return r;
-    if (!l2 || !linf) return fail(ctx, HC_ERR_ARG, "null output");
-    r = residual_to_device(ctx, k);
-    if (r) return r;
     const CaseHost& c = ctx->cases[k];
     hipLaunchKernelGGL(k_norm_partials, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream,
                        ctx->res + (ctx->nv + 1) * c.cell_off, ctx->cvol + c.cell_off, c.N, ctx->nv, ctx->part);
@@ -752,6 +759,16 @@ int norms(hc_context* ctx, int k, double* l2, double* linf)
         linf[v] = out[4 + v];
     }
     return check_status(ctx);
+}
+
+int norms(hc_context* ctx, int k, double* l2, double* linf)
+{
+    int r = case_ready(ctx, k);
+    if (r) return r;
+    if (!l2 || !linf) return fail(ctx, HC_ERR_ARG, "null output");
+    r = residual_to_device(ctx, k);
+    if (r) return r;
+    return norms_of_res(ctx, k, l2, linf);
 }
 
 int get_time(hc_context* ctx, int k, double* t)
@@ -1050,6 +1067,26 @@ int hc_batch_get_state(hc_context* ctx, int k, double* q) { return get_state(ctx
 int hc_batch_residual(hc_context* ctx, int k, double* res) { return residual(ctx, k, res); }
 int hc_batch_norms(hc_context* ctx, int k, double* l2, double* linf) { return norms(ctx, k, l2, linf); }
 int hc_batch_get_time(hc_context* ctx, int k, double* t) { return get_time(ctx, k, t); }
+
+int hc_step_norms(hc_context* ctx, int nsteps, double* l2, double* linf)
+{
+    int r = ready(ctx);
+    if (r) return r;
+    if (nsteps < 1 || !l2 || !linf) return fail(ctx, HC_ERR_ARG, "hc_step_norms: nsteps >= 1 and outputs");
+    if (ctx->cases.size() != 1) return fail(ctx, HC_ERR_STATE, "hc_step_norms drives a single-case context");
+    if ((r = hc_step(ctx, nsteps - 1))) return r;
+    if (ctx->smooth_mode >= 0) {
+        // the smoothed step has no fused residual store: the same residual, evaluated once more
+        if ((r = hc_norms(ctx, l2, linf))) return r;
+        return hc_step(ctx, 1);
+    }
+    if ((r = all_states(ctx))) return r;
+    ctx->keep_res = true;   // the last step, eager: its stage 0 stores the residual of its start state
+    r = enqueue_step(ctx);
+    ctx->keep_res = false;
+    if (r) return r;
+    return norms_of_res(ctx, 0, l2, linf);
+}
 
 int hc_step_steady(hc_context* ctx, long long max_steps, double residual_drop, int check_every,
                    long long* steps_done, double* drop)

@@ -100,6 +100,7 @@ struct hc_context {
     double* blk = nullptr;
     size_t blk_n = 0;
     // residual smoothing of the stepping (hc_set_residual_smoothing): mode -1 off, its buffers
+    bool keep_res = false;         // enqueue_step: stage 0 also stores its residual in res (hc_step_norms)
     int smooth_mode = -1;
     double smooth_alpha = 1.0;
     double* sm = nullptr;

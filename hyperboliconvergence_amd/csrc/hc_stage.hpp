@@ -124,6 +124,7 @@ template <class RF> struct StageArgs {
     int vscale, pscale, limiter;
     int bc[4];
     int mode, dt_mode, stage, reset_max;
+    int keep_res;         // fused stages: also store the stage's residual in res_out (stage 0 of a step)
     long long N;
     double cfl;
     const double* q_in;   // stage state [nv][N]
@@ -646,10 +647,10 @@ struct StageTile {
             else minus(kjr);
         }
         const long long c = (long long)i * nj + j;
-        if (a.mode == MODE_RESIDUAL) {
+        if (a.mode == MODE_RESIDUAL || a.keep_res) {
             for (int v = 0; v < NV; ++v) a.res_out[c * (NV + 1) + v] = R[v];
             a.res_out[c * (NV + 1) + NV] = lam;
-            return;
+            if (a.mode == MODE_RESIDUAL) return;
         }
         // Runge-Kutta accumulation (rungeKutta.h:176-197)
         const RkPlan& p = a.plan;
@@ -768,6 +769,7 @@ template <class RF> struct LaunchArgs {
     int nslots;
     int vscale, pscale, limiter;
     int mode, dt_mode, stage, reset_max;
+    int keep_res;         // fused stages: also store the stage's residual in res_out (stage 0 of a step)
     double cfl;
     RkPlan plan;
     const double* q_in;   // packed [nv][N_c] blocks
@@ -798,6 +800,7 @@ template <class RF, int NV> HC_HD StageArgs<RF> case_args(const LaunchArgs<RF>& 
     a.dt_mode = L.dt_mode;
     a.stage = L.stage;
     a.reset_max = L.reset_max;
+    a.keep_res = L.keep_res;
     a.N = c.N;
     a.cfl = L.cfl;
     a.q_in = L.q_in + NV * c.cell_off;

@@ -151,6 +151,15 @@ int hc_get_time(hc_context* ctx, double* t);
  */
 int hc_norms(hc_context* ctx, double* l2, double* linf);
 
+/*
+ * hc_step_norms: nsteps steps (>= 1) as hc_step, and the norms (as hc_norms) of the residual of
+ * the state the last step started from, which that step's first stage computes anyway: the stage
+ * kernel also stores it and a reduction pass folds it, instead of a separate residual evaluation.
+ * Single-case contexts.  The state afterwards is hc_step's; the norms equal hc_step(nsteps - 1)
+ * followed by hc_norms.  No counterpart in the reference.
+ */
+int hc_step_norms(hc_context* ctx, int nsteps, double* l2, double* linf);
+
 /* OR of the status bits raised by the device since the last call (cleared on read). */
 int hc_get_status(hc_context* ctx, unsigned int* status);
 

@@ -9,11 +9,21 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 _L = None
 
 
+def make_locked(targets):
+    """make the given targets under an exclusive file lock: parallel test workers must not load a
+    library that another worker is rewriting."""
+    import fcntl
+    with open(os.path.join(HERE, ".make.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        return subprocess.run(["make", "-C", HERE, "-s", *targets], capture_output=True, text=True, timeout=900)
+
+
 def lib():
     global _L
     if _L is None:
         path = os.path.join(HERE, "libhostemu.so")
-        subprocess.run(["make", "-C", HERE, "-s"], check=True, capture_output=True)
+        r = make_locked(["libhostemu.so"])
+        assert r.returncode == 0, r.stderr[-2000:]
         _L = C.CDLL(path)
         P, i = C.c_void_p, C.c_int
         _L.emu_residual.argtypes = [P, i, i, P, P, P, i, P, P]

@@ -324,6 +324,31 @@ def test_norms_match_oracle():
     assert np.array_equal(linf, linfo)
 
 
+@pytest.mark.parametrize("which", ["cylinder-local-dt", "vortex-global-dt", "cylinder-smoothed"])
+@pytest.mark.parametrize("graph_steps", [0, 4])
+def test_step_norms_fused_into_the_last_step_equal_step_then_norms(which, graph_steps):
+    """hc_step_norms(9): the state of hc_step(9), and the norms of the residual the last step
+    starts from (stored by its first stage) equal hc_step(8) then hc_norms, bit for bit."""
+    cfg = _vortex_cfg(64, 4) if which.startswith("vortex") else _cyl_cfg(64, 32, perturb=0.02)
+    res = {}
+    for fused in (True, False):
+        with _ctx(cfg) as ctx:
+            ctx.set_tiles(0)
+            ctx.set_graph_steps(graph_steps)
+            if which.endswith("smoothed"):
+                ctx.set_residual_smoothing(1, 2.0)
+            if fused:
+                l2, linf = ctx.step_norms(9)
+            else:
+                ctx.step(8)
+                l2, linf = ctx.norms()
+                ctx.step(1)
+            res[fused] = (l2, linf, ctx.get_state(), ctx.status())
+    for a, b in zip(res[True], res[False]):
+        assert np.array_equal(a, b)
+    assert res[True][3] == 0 and np.all(res[True][0] > 0)
+
+
 def test_deterministic_repeat():
     cfg = _vortex_cfg(64, 3)
     outs = []

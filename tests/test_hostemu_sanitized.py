@@ -24,7 +24,7 @@ NTILE = 3
 @pytest.fixture(scope="module")
 def driver():
     # make rebuilds only when a kernel source changed (a full build takes a few minutes)
-    r = subprocess.run(["make", "-s", "asan_driver"], cwd=HERE, capture_output=True, text=True, timeout=900)
+    r = pyemu.make_locked(["asan_driver"])
     if r.returncode != 0 and "sanitize" in r.stderr and "cannot find" in r.stderr:
         pytest.skip("this compiler has no sanitizer runtime")
     assert r.returncode == 0, r.stderr[-2000:]
