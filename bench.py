@@ -187,7 +187,7 @@ def batch_cfgs(flux=None):
             for r in cases.batch_case_table()]
 
 
-def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True):
+def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True, tile_order=1):
     """Time `steps` steps of the batch from the initial states after `warmup` untimed steps.
     Set-up outside the timed region and outside the warm-up count: the tile-shape autotuning and
     the capture of the step graph (one graph's worth of steps, then the states are reset)."""
@@ -198,6 +198,8 @@ def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True):
         b.add_case(cfg["nodes"], cfg["case"])
     b.end()
     b.set_precision(precision)
+    b.set_tile_order(tile_order)
+    ngeo = b.geometry_blocks()
     k = min(graph_steps, steps) if graph_steps > 0 else 0
     b.set_graph_steps(k)
 
@@ -226,7 +228,7 @@ def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True):
     qs = [b.get_case_state(c) for c in range(len(cfgs))] if keep_states else None
     st = b.status()
     b.close()
-    return t1 - t0, qs, st, region, tiles
+    return t1 - t0, qs, st, region, tiles, ngeo
 
 
 def batch_states(cfgs, precision, steps, graph_steps=50):
@@ -319,7 +321,8 @@ def main_batch(a):
     ns = cfgs[0]["case"].nstages
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
-        el, qs, st, region, tiles = run_batch(cfgs, prec, steps, a.warmup, a.graph_steps)
+        el, qs, st, region, tiles, ngeo = run_batch(cfgs, prec, steps, a.warmup, a.graph_steps,
+                                                    tile_order=a.tile_order)
         res[prec] = dict(seconds=el, qs=qs, status=st, kernel_ms=region[0] / max(region[1], 1.0),
                          rate=Ntot * ns * steps / el, tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]])
         print(f"# batch {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, {res[prec]['rate']:.4e} "
@@ -345,18 +348,28 @@ def main_batch(a):
         cfgs_ref = batch_cfgs(hca.FLUX_ROE)
         ref = {"flux": "RoeFlux", "steps": 50}
         for prec in (0, 1):
-            el, _, st, _, _ = run_batch(cfgs_ref, prec, 50, a.warmup, a.graph_steps, keep_states=False)
+            el, _, st, _, _, _ = run_batch(cfgs_ref, prec, 50, a.warmup, a.graph_steps, keep_states=False,
+                                           tile_order=a.tile_order)
             ref["mixed_value" if prec else "fp64_value"] = Ntot * ns * 50 / el
             ref["mixed_status" if prec else "fp64_status"] = int(st)
         print(f"# batch reference Roe over 50 steps: fp64 {ref['fp64_value']:.4e}, mixed {ref['mixed_value']:.4e} "
               f"cell-stage/s", file=sys.stderr, flush=True)
     with hca.Context(0) as c:
         triad = c.stream_triad(1 << 27, 20)
+    # compulsory bytes per cell-stage of a curvilinear case: the stage state read, the step-start
+    # state read and the new state written (3 x 4 x 8 B) plus one FP64 node of geometry (16 B).
+    # Cases on the same mesh share one geometry copy, so counting the geometry once per distinct
+    # mesh cell gives the smaller figure reported beside it.
     bpc = 112.0
+    shapes = {c["q"].shape[:2] for c in cfgs}
+    geo_cells = sum(s0 * s1 for s0, s1 in shapes) if ngeo == len(shapes) else Ntot
+    bpc_shared = 96.0 + 16.0 * geo_cells / Ntot
     achieved = bpc * Ntot / (h["kernel_ms"] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("c4", head, h["tiles"]),
-                "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bpc, "kernel_us": h["kernel_ms"] * 1e3}
+                "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bpc, "kernel_us": h["kernel_ms"] * 1e3,
+                "alg_bytes_per_cell_stage_shared_mesh": bpc_shared,
+                "frac_shared_mesh": bpc_shared * Ntot / (h["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
     cpu = None if a.no_cpu_baseline else cpu_baseline_batch(cfgs)
     if cpu:
         print(f"# cpu baseline: {cpu['value']:.4e} cell-stage/s on {cpu['cores']} threads, "
@@ -370,7 +383,8 @@ def main_batch(a):
                    "cells": Ntot, "rk_stages": ns, "configured_steps": HORIZON_C4,
                    "precision": "mixed" if head == 1 else "fp64",
                    "graph_steps": min(a.graph_steps, steps), "tile_shape": h["tiles"],
-                   "tile_shape_ms_per_step": h["tile_ms"]},
+                   "tile_shape_ms_per_step": h["tile_ms"], "geometry_blocks": ngeo,
+                   "tile_order": "interleaved" if a.tile_order else "case by case"},
         "roofline": roofline, "cpu_baseline": cpu,
         "fp64_value": res[0]["rate"] if 0 in res else None, "mixed_value": res[1]["rate"] if 1 in res else None,
         "mixed_vs_fp64_rel_err": mixed_err, "mixed_vs_fp64_per_case": per_case, "status": int(h["status"]),
@@ -507,6 +521,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the divergence search and the face-frame Roe runs (for counter collection)")
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--tile-order", type=int, default=1, choices=[0, 1],
+                    help="c4: 1 interleaves the tiles of cases sharing a mesh in a step, 0 runs case by case")
     ap.add_argument("--uniform-geometry", type=int, default=1, choices=[0, 1],
                     help="1: uniform-geometry stage kernels where the mesh allows them (c1, c3); 0: general kernels")
     a = ap.parse_args()

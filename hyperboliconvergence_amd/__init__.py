@@ -32,6 +32,7 @@ C_ABI_SYMBOLS = (
     "hc_batch_begin", "hc_batch_add_case", "hc_batch_end", "hc_batch_size", "hc_batch_set_boundary_refs",
     "hc_batch_set_state", "hc_batch_get_state", "hc_batch_residual", "hc_batch_norms", "hc_batch_get_time",
     "hc_set_tiles", "hc_get_tiles", "hc_set_uniform_geometry", "hc_get_uniform_geometry",
+    "hc_set_tile_order", "hc_get_geometry_blocks",
     "hc_tangent_residual", "hc_adjoint_residual", "hc_adjoint_forward", "hc_adjoint_backward", "hc_tangent_steps",
     "hc_set_state_device", "hc_get_state_device", "hc_set_boundary_refs_device", "hc_adjoint_backward_device",
     "hc_step_norms",
@@ -115,6 +116,8 @@ def library():
         "hc_get_tiles": (i, [P, P, P]),
         "hc_set_uniform_geometry": (i, [P, i]),
         "hc_get_uniform_geometry": (i, [P, P]),
+        "hc_set_tile_order": (i, [P, i]),
+        "hc_get_geometry_blocks": (i, [P, P]),
         "hc_tangent_residual": (i, [P, P, P, d, P, P]),
         "hc_adjoint_residual": (i, [P, P, P, P, P]),
         "hc_adjoint_forward": (i, [P, i, i]),
@@ -281,6 +284,17 @@ class Context:
         act = C.c_int()
         self._check(self._L.hc_get_uniform_geometry(self._h, C.byref(act)), "hc_get_uniform_geometry")
         return bool(act.value)
+
+    def set_tile_order(self, interleave):
+        """1 (default): a step runs the cases that share a geometry copy tile by tile together;
+        0: case after case.  Bit-identical results."""
+        self._check(self._L.hc_set_tile_order(self._h, int(interleave)), "hc_set_tile_order")
+
+    def geometry_blocks(self):
+        """Number of geometry copies on the device (cases with bitwise-equal geometry share one)."""
+        n = C.c_int()
+        self._check(self._L.hc_get_geometry_blocks(self._h, C.byref(n)), "hc_get_geometry_blocks")
+        return n.value
 
     # ---- derivatives (single case, FP64) ----
     def _brows(self):

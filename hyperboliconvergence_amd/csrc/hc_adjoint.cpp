@@ -14,6 +14,7 @@
 #include "hc_context.hpp"
 
 #include <cstring>
+#include <utility>
 
 using namespace hc;
 using namespace hc_detail;
@@ -70,19 +71,28 @@ __global__ void __launch_bounds__(256) k_sum_final(const double* part, int n, do
     if (threadIdx.x == 0) out[slot] = accumulate ? out[slot] + s[0] : s[0];
 }
 
-// y += x
-__global__ void __launch_bounds__(256) k_add(double* y, const double* x, long long n)
+// Runge-Kutta adjoint of one stage, given x = the adjoint of the stage's output state:
+//   lam (+)= x  and  yb_k (+)= (coef_k / V_c) x  for the residuals k the stage combines,
+// each target assigned on its first contribution of the step and added to afterwards
+struct RkAdj {
+    double* yb[6];
+    double coef[6];
+    int add[6];
+    int nk;
+    int lam_add;
+};
+__global__ void __launch_bounds__(256) k_rk_adjoint(double* lam, const double* x, RkAdj t, const double* vol,
+                                                    long long N, int nv)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < n) y[i] = y[i] + x[i];
-}
-
-// y[v][c] += coef / vol[c] * x[v][c]   (SoA [nv][N])
-__global__ void __launch_bounds__(256) k_add_scaled_vol(double* y, const double* x, double coef, const double* vol,
-                                                        long long N, int nv)
-{
-    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i < N * nv) y[i] = y[i] + (coef / vol[i % N]) * x[i];
+    if (i >= N * nv) return;
+    const double xi = x[i];
+    lam[i] = t.lam_add ? lam[i] + xi : xi;
+    const double v = vol[i % N];
+    for (int k = 0; k < t.nk; ++k) {
+        const double y = (t.coef[k] / v) * xi;
+        t.yb[k][i] = t.add[k] ? t.yb[k][i] + y : y;
+    }
 }
 
 // stage tangent: out[v][c] = base[v][c] + (beta dt / V_c) sum_k coef[k] rd_k[c][v]
@@ -246,19 +256,20 @@ int download_soa(hc_context* ctx, double* aos, const double* src, long long N, i
     return HC_OK;
 }
 
-// adjoint residual at state q (device SoA) with ybar (device SoA): qbar in s.qbar; minf_bar
-// added to s.scal[slot_minf]; boundary-reference adjoint added to s.brefbar_tot
-int adjoint_at(hc_context* ctx, const double* q, const double* ybar)
+// adjoint residual at state q (device SoA) with ybar (device SoA): qbar delivered into dst (assigned,
+// or added when add is set); minf_bar added to s.scal[0]; the boundary-reference adjoint added to
+// s.brefbar_tot
+int adjoint_at(hc_context* ctx, const double* q, const double* ybar, double* dst, bool add)
 {
     AdjState& s = *ctx->adj;
-    const auto& c = ctx->cases[0];
     AdjArgs a = base_adj(ctx);
     a.q = q;
     a.ybar = ybar;
-    const long long nb = (long long)c.brows * s.nv;
-    HC_HIP(hipMemsetAsync(s.brefbar, 0, nb * sizeof(double), ctx->stream));
+    a.brefbar = s.brefbar_tot;
+    a.bref_acc = 1;
+    a.qacc = dst;
+    a.qacc_add = add ? 1 : 0;
     HC_HIP(run_adjoint(ctx, a));
-    hipLaunchKernelGGL(k_add, dim3(blocks(nb)), dim3(256), 0, ctx->stream, s.brefbar_tot, s.brefbar, nb);
     hipLaunchKernelGGL(k_sum_partials, dim3(SUM_BLOCKS), dim3(256), 0, ctx->stream, s.minf_face, s.nfaces, s.part);
     hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, ctx->stream, s.part, SUM_BLOCKS, s.scal, 0, 1);
     HC_HIP(hipGetLastError());
@@ -354,7 +365,7 @@ int hc_adjoint_residual(hc_context* ctx, const double* ybar, double* qbar, doubl
     const long long nb = (long long)c.brows * s.nv;
     HC_HIP(hipMemsetAsync(s.brefbar_tot, 0, nb * sizeof(double), ctx->stream));
     HC_HIP(hipMemsetAsync(s.scal, 0, 4 * sizeof(double), ctx->stream));
-    if ((r = adjoint_at(ctx, ctx->q[ctx->cur], s.ybar_in))) return r;
+    if ((r = adjoint_at(ctx, ctx->q[ctx->cur], s.ybar_in, s.qbar, false))) return r;
     if ((r = download_soa(ctx, qbar, s.qbar, s.N, s.nv))) return r;
     HC_HIP(hipMemcpy(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToHost));
     HC_HIP(hipMemcpy(minf_bar, s.scal, sizeof(double), hipMemcpyDeviceToHost));
@@ -399,14 +410,17 @@ static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* q
     const int ns = s.ns, nv = s.nv;
     const long long N = s.N, len = (long long)nv * N, nb = (long long)c.brows * nv;
     const double dt = ctx->cs.cfl;
-    if ((r = grow(ctx, s.seg, s.C, len)) || (r = grow(ctx, s.stage, ns, len)) || (r = grow(ctx, s.a, ns, len)) ||
-        (r = grow(ctx, s.yb, ns, len)))
+    // every stage state of a segment is kept (C steps x ns stages), so the reverse sweep recomputes
+    // each step once, from the segment's checkpoint
+    if ((r = grow(ctx, s.seg, (size_t)s.C * ns, len)) || (r = grow(ctx, s.a, ns + 1, len)) || (r = grow(ctx, s.yb, ns, len)))
         return r;
-    // qbn holds the adjoint of the state at the current step boundary
+    // lam holds the adjoint of the state at the current step boundary; nxt receives the one before
+    double* lam = s.qbn;
+    double* nxt = s.a[ns];
     if (dev) {
         HC_HIP(hipStreamSynchronize(ctx->stream));
-        if ((r = aos_to_soa_device(ctx, s.qbn, qbar_final, N, nv))) return r;
-    } else if ((r = upload_soa(ctx, s.qbn, qbar_final, N, nv))) {
+        if ((r = aos_to_soa_device(ctx, lam, qbar_final, N, nv))) return r;
+    } else if ((r = upload_soa(ctx, lam, qbar_final, N, nv))) {
         return r;
     }
     HC_HIP(hipMemsetAsync(s.brefbar_tot, 0, nb * sizeof(double), ctx->stream));
@@ -415,53 +429,57 @@ static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* q
     for (int g = nseg - 1; g >= 0; --g) {
         const int n0 = g * s.C;
         const int n = std::min(s.C, s.K - n0);
-        // states n0 .. n0+n-1 of the segment from its checkpoint
+        // the stage states of steps n0 .. n0+n-1 from the segment's checkpoint: seg[t ns + st] is the
+        // input of stage st of step t (st = 0: the step's start state)
         if ((r = copy_d(ctx, s.seg[0], s.ckpt[g], len))) return r;
-        for (int t = 1; t < n; ++t) {
-            // one step from seg[t-1] into seg[t]: stages into stage buffers, last stage into seg[t]
-            const double* in = s.seg[t - 1];
+        for (int t = 0; t < n; ++t) {
+            const double* q0 = s.seg[(size_t)t * ns];
+            const double* in = q0;
             for (int st = 0; st < ns; ++st) {
-                double* out = st == ns - 1 ? s.seg[t] : s.stage[st + 1];
-                if ((r = stage_into(ctx, st, in, s.seg[t - 1], out))) return r;
+                if (st == ns - 1 && t + 1 == n) break;   // the next segment's checkpoint is not needed
+                double* out = st == ns - 1 ? s.seg[(size_t)(t + 1) * ns] : s.seg[(size_t)t * ns + st + 1];
+                if ((r = stage_into(ctx, st, in, q0, out))) return r;
                 in = out;
             }
         }
         for (int t = n - 1; t >= 0; --t) {
-            const double* qn = s.seg[t];
-            // stage states q_1 .. q_{ns-1}
-            const double* in = qn;
-            for (int st = 0; st + 1 < ns; ++st) {
-                if ((r = stage_into(ctx, st, in, qn, s.stage[st + 1]))) return r;
-                in = s.stage[st + 1];
-            }
-            for (int k = 0; k < ns; ++k) {
-                HC_HIP(hipMemsetAsync(s.a[k], 0, len * sizeof(double), ctx->stream));
-                HC_HIP(hipMemsetAsync(s.yb[k], 0, len * sizeof(double), ctx->stream));
-            }
-            if ((r = copy_d(ctx, s.a[ns - 1], s.qbn, len))) return r;
-            HC_HIP(hipMemsetAsync(s.qbn, 0, len * sizeof(double), ctx->stream));
+            // a[st]: adjoint of stage st's output (the last stage's output is the step's end state)
+            double* a[6];
+            for (int st = 0; st + 1 < ns; ++st) a[st] = s.a[st];
+            a[ns - 1] = lam;
+            bool yb_set[6] = {false, false, false, false, false, false};
             for (int st = ns - 1; st >= 0; --st) {
-                hipLaunchKernelGGL(k_add, dim3(blocks(len)), dim3(256), 0, ctx->stream, s.qbn, s.a[st], len);
+                RkAdj t_adj{};
+                t_adj.lam_add = st < ns - 1 ? 1 : 0;
                 for (int k = 0; k <= st; ++k) {
                     const double alpha = ctx->cs.alpha[st][k];
                     if (alpha == 0.0) continue;
-                    hipLaunchKernelGGL(k_add_scaled_vol, dim3(blocks(len)), dim3(256), 0, ctx->stream, s.yb[k], s.a[st],
-                                       alpha * (ctx->cs.beta[st] * dt), ctx->cvol, N, nv);
+                    t_adj.yb[t_adj.nk] = s.yb[k];
+                    t_adj.coef[t_adj.nk] = alpha * (ctx->cs.beta[st] * dt);
+                    t_adj.add[t_adj.nk] = yb_set[k] ? 1 : 0;
+                    t_adj.nk++;
+                    yb_set[k] = true;
                 }
+                hipLaunchKernelGGL(k_rk_adjoint, dim3(blocks(len)), dim3(256), 0, ctx->stream, nxt, a[st], t_adj,
+                                   ctx->cvol, N, nv);
                 HC_HIP(hipGetLastError());
-                if ((r = adjoint_at(ctx, st == 0 ? qn : s.stage[st], s.yb[st]))) return r;
-                double* dst = st > 0 ? s.a[st - 1] : s.qbn;
-                hipLaunchKernelGGL(k_add, dim3(blocks(len)), dim3(256), 0, ctx->stream, dst, s.qbar, len);
-                HC_HIP(hipGetLastError());
+                if (!yb_set[st]) {   // no stage combines R_st: its adjoint weight is zero
+                    HC_HIP(hipMemsetAsync(s.yb[st], 0, len * sizeof(double), ctx->stream));
+                    yb_set[st] = true;
+                }
+                // the adjoint residual of stage st feeds the previous stage's output, or the step's start
+                if ((r = adjoint_at(ctx, s.seg[(size_t)t * ns + st], s.yb[st], st > 0 ? a[st - 1] : nxt, st == 0)))
+                    return r;
             }
+            std::swap(lam, nxt);
         }
     }
     if (dev) {
-        if ((r = soa_to_aos_device(ctx, qbar_initial, s.qbn, N, nv))) return r;
+        if ((r = soa_to_aos_device(ctx, qbar_initial, lam, N, nv))) return r;
         HC_HIP(hipMemcpyAsync(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
         HC_HIP(hipStreamSynchronize(ctx->stream));
     } else {
-        if ((r = download_soa(ctx, qbar_initial, s.qbn, N, nv))) return r;
+        if ((r = download_soa(ctx, qbar_initial, lam, N, nv))) return r;
         HC_HIP(hipMemcpy(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToHost));
     }
     HC_HIP(hipMemcpy(minf_bar, s.scal, sizeof(double), hipMemcpyDeviceToHost));

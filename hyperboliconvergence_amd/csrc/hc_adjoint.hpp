@@ -10,9 +10,12 @@
 //             Rdot = dR/dq qdot + dR/dminf minf_dot + dR/dbref bref_dot, by evaluating the
 //             point functions of hc_point.hpp on Dual<1> numbers.
 //   adjoint:  given ybar, the kernels return qbar = (dR/dq)^T ybar, minf_bar and bref_bar.
-//             Per face, the flux Jacobian with respect to the reconstructed left/right
-//             states (and minf, or the boundary reference state) comes from Dual<9>
-//             numbers and is applied transposed; the MUSCL reconstruction (limiter
+//             Per face, the transposed flux Jacobian with respect to the reconstructed
+//             left/right states (and minf, or the boundary reference state) is applied to
+//             the face's adjoint weight: AUSM+up on interior faces by its hand-written
+//             reverse sweep (hc_flux_vjp.hpp), the other fluxes and the boundary faces
+//             from Dual<9> numbers (boundary faces in a kernel of their own, so that the
+//             interior kernel does not hold the Dual<9> registers); the MUSCL reconstruction (limiter
 //             derivatives from Dual<2>, the LSQ-biased slopes, which are linear in the
 //             LSQ solution metric Q) and the metric Q = sum (q_R - q_L) dx are then
 //             transposed by hand: faces write their contributions, cells gather them in
@@ -26,6 +29,7 @@
 #pragma once
 
 #include "hc_dual.hpp"
+#include "hc_flux_vjp.hpp"
 #include "hc_stage.hpp"
 
 namespace hc {
@@ -53,6 +57,9 @@ struct AdjArgs {
     double* brefbar;       // [rows][nv] (rows without a face stay as they are)
     double* qbar;          // [nv][N]
     double* Qbar;          // [nv][2][N]
+    int bref_acc;          // 1: boundary faces add their reference-state adjoint to brefbar
+    double* qacc;          // if set, adj_qedges delivers qbar into it instead of qbar:
+    int qacc_add;          //   qacc = qbar (0) or qacc = qacc + qbar (1)
 };
 
 // geometry and cells of one face
@@ -339,7 +346,72 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
     // cell's contribution on the side that cell occupies (right of node line 0, left of the last).
     HC_HD static void adj_face(const AdjArgs& a, long long f)
     {
+        if (face_ref(a.m, f).boundary)
+            for (int k = 0; k < NDIR; ++k) adj_face_boundary(a, f, k);
+        else
+            adj_face_interior(a, f);
+    }
+
+    // boundary-face index b in [0, 2 nj + 2 ni) -> face index (i-faces of node lines 0 and ni, then
+    // j-faces of node lines 0 and nj), whatever the directions' periodicity
+    HC_HD static long long boundary_face(const MeshView<double>& m, long long b)
+    {
+        const long long ni = m.ni, nj = m.nj;
+        if (b < nj) return b;
+        if (b < 2 * nj) return ni * nj + (b - nj);
+        if (b < 2 * nj + ni) return m.nfi + (b - 2 * nj) * (nj + 1);
+        return m.nfi + (b - 2 * nj - ni) * (nj + 1) + nj;
+    }
+
+    // Non-periodic boundary face, direction k of its transposed flux Jacobian: k < NV the interior
+    // state, k < 2 NV the boundary reference state, k = 2 NV minf; one Dual<1> evaluation per
+    // direction (each direction's derivative has the bits it has in a wider dual evaluation).
+    // The directions of a face write disjoint outputs: k < NV its cell's row and a zero in the
+    // other side's row, NV <= k < 2 NV four of the Qbar rows (zeros) and the reference adjoint,
+    // k = 2 NV the face's minf adjoint.  Other faces are left alone.
+    static constexpr int NDIR = 2 * NV + 1;
+    HC_HD static void adj_face_boundary(const AdjArgs& a, long long f, int k)
+    {
         const FaceRef r = face_ref(a.m, f);
+        if (!r.valid || !r.boundary) return;
+        const long long nF = a.nfaces;
+        const double* base = r.isi ? a.m.fi : a.m.fj;
+        const long long nf = r.isi ? a.m.nfi : a.m.nfj;
+        const long long N = a.N;
+        const int type = bc_of(a, r.bid);
+        const bool plus = (r.bid == 0 || r.bid == 2) || type == BC_WALL;
+        const int side = (r.bid == 0 || r.bid == 2) ? NV : 0;   // interior cell right / left of the face
+        if (k < NV) a.fout[(NV - side + k) * nF + f] = 0.0;
+        else if (k < 2 * NV)
+            for (int m = 0; m < 4; ++m) a.fout[(2 * NV + 4 * (k - NV) + m) * nF + f] = 0.0;
+        const Species<D1> sp1 = dual_species<1>(a.sp, k == 2 * NV ? 0 : -1);
+        D1 qi[NV], qf[NV];
+        for (int v = 0; v < NV; ++v) {
+            qi[v] = make_dual<1>(a.q[v * N + r.cl], v == k ? 0 : -1);
+            qf[v] = make_dual<1>(a.bref[r.row * NV + v], NV + v == k ? 0 : -1);
+        }
+        const Flux4<D1> F = boundary_flux<D1, LAW, BASIS, FLUX>(sp1, a.vscale, a.pscale, type, r.bid, base, nf, r.fidx,
+                                                                 qi, qf, a.m.ccx[r.cl], a.m.ccy[r.cl]);
+        double g = 0.0;
+        for (int v = 0; v < NV; ++v) {
+            const double Fb = plus ? a.ybar[v * N + r.cl] : -a.ybar[v * N + r.cl];
+            g = g + Fb * F.f[v].d[0];
+        }
+        if (k < NV) {
+            a.fout[(side + k) * nF + f] = g;
+        } else if (k < 2 * NV) {
+            double* dst = a.brefbar + r.row * NV + (k - NV);
+            if (type != BC_WALL) *dst = a.bref_acc ? *dst + g : g;
+        } else {
+            a.minf_face[f] = g;
+        }
+    }
+
+    // interior or periodic face (boundary faces are left to adj_face_boundary)
+    HC_HD static void adj_face_interior(const AdjArgs& a, long long f)
+    {
+        const FaceRef r = face_ref(a.m, f);
+        if (r.boundary) return;
         const long long nF = a.nfaces;
         for (int k = 0; k < 6 * NV; ++k) a.fout[k * nF + f] = 0.0;
         a.minf_face[f] = 0.0;
@@ -347,34 +419,7 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
         const double* base = r.isi ? a.m.fi : a.m.fj;
         const long long nf = r.isi ? a.m.nfi : a.m.nfj;
         const long long N = a.N;
-        const Species<D9> sp9 = dual_species<9>(a.sp, 8);
         double Fb[NV];
-        if (r.boundary) {
-            const int type = bc_of(a, r.bid);
-            const bool plus = (r.bid == 0 || r.bid == 2) || type == BC_WALL;
-            for (int v = 0; v < NV; ++v) Fb[v] = plus ? a.ybar[v * N + r.cl] : -a.ybar[v * N + r.cl];
-            D9 qi[NV], qf[NV];
-            for (int v = 0; v < NV; ++v) {
-                qi[v] = make_dual<9>(a.q[v * N + r.cl], v);
-                qf[v] = make_dual<9>(a.bref[r.row * NV + v], 4 + v);
-            }
-            const Flux4<D9> F = boundary_flux<D9, LAW, BASIS, FLUX>(sp9, a.vscale, a.pscale, type, r.bid, base, nf,
-                                                                     r.fidx, qi, qf, a.m.ccx[r.cl], a.m.ccy[r.cl]);
-            const int side = (r.bid == 0 || r.bid == 2) ? NV : 0;   // interior cell right / left of the face
-            double mb = 0.0;
-            for (int k = 0; k < NV; ++k) {
-                double gi = 0.0, gf = 0.0;
-                for (int v = 0; v < NV; ++v) {
-                    gi = gi + Fb[v] * F.f[v].d[k];
-                    gf = gf + Fb[v] * F.f[v].d[4 + k];
-                }
-                a.fout[(side + k) * nF + f] = gi;
-                if (type != BC_WALL) a.brefbar[r.row * NV + k] = gf;
-            }
-            for (int v = 0; v < NV; ++v) mb = mb + Fb[v] * F.f[v].d[8];
-            a.minf_face[f] = mb;
-            return;
-        }
         for (int v = 0; v < NV; ++v) Fb[v] = a.ybar[v * N + r.cr] - a.ybar[v * N + r.cl];
         const double area = base[FS_AREA * nf + r.fidx];
         const double n0 = base[FS_N0 * nf + r.fidx];
@@ -419,28 +464,34 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
                 qrf[v] = qr[v] - sr.v * dr;
             }
         }
-        // flux Jacobian with respect to the reconstructed states and minf
-        FaceG<D9> fg;
-        fg.area = D9(area);
-        fg.n0 = D9(n0);
-        fg.n1 = D9(n1);
-        D9 xl[NV], xr[NV];
-        for (int v = 0; v < NV; ++v) {
-            xl[v] = make_dual<9>(qlf[v], v);
-            xr[v] = make_dual<9>(qrf[v], 4 + v);
-        }
-        const Flux4<D9> F = face_flux<D9, LAW, BASIS, FLUX>(sp9, a.vscale, a.pscale, fg, xl, xr);
+        // transposed flux Jacobian (reconstructed states and minf) applied to Fb
         double qlb[NV], qrb[NV], mb = 0.0;
-        for (int k = 0; k < NV; ++k) {
-            double gl = 0.0, gr = 0.0;
+        if constexpr (LAW == LAW_EULER && FLUX == FLUX_AUSMPUP) {
+            for (int v = 0; v < NV; ++v) qlb[v] = qrb[v] = 0.0;
+            mb = ausmpup_vjp<BASIS>(a.sp, a.vscale, a.pscale, FaceG<double>{area, n0, n1}, qlf, qrf, Fb, qlb, qrb);
+        } else {
+            const Species<D9> sp9 = dual_species<9>(a.sp, 8);
+            FaceG<D9> fg;
+            fg.area = D9(area);
+            fg.n0 = D9(n0);
+            fg.n1 = D9(n1);
+            D9 xl[NV], xr[NV];
             for (int v = 0; v < NV; ++v) {
-                gl = gl + Fb[v] * F.f[v].d[k];
-                gr = gr + Fb[v] * F.f[v].d[4 + k];
+                xl[v] = make_dual<9>(qlf[v], v);
+                xr[v] = make_dual<9>(qrf[v], 4 + v);
             }
-            qlb[k] = gl;
-            qrb[k] = gr;
+            const Flux4<D9> F = face_flux<D9, LAW, BASIS, FLUX>(sp9, a.vscale, a.pscale, fg, xl, xr);
+            for (int k = 0; k < NV; ++k) {
+                double gl = 0.0, gr = 0.0;
+                for (int v = 0; v < NV; ++v) {
+                    gl = gl + Fb[v] * F.f[v].d[k];
+                    gr = gr + Fb[v] * F.f[v].d[4 + k];
+                }
+                qlb[k] = gl;
+                qrb[k] = gr;
+            }
+            for (int v = 0; v < NV; ++v) mb = mb + Fb[v] * F.f[v].d[8];
         }
-        for (int v = 0; v < NV; ++v) mb = mb + Fb[v] * F.f[v].d[8];
         a.minf_face[f] = mb;
         if (!recon) {
             for (int v = 0; v < NV; ++v) {
@@ -522,7 +573,11 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
         else if (a.m.per_j) edge((long long)gi * nj + nj - 1, c, a.m.fj, a.m.nfj, (long long)gi * (nj + 1), 1.0);
         if (gj < nj - 1) edge(c, c + 1, a.m.fj, a.m.nfj, (long long)gi * (nj + 1) + gj + 1, -1.0);
         else if (a.m.per_j) edge(c, (long long)gi * nj, a.m.fj, a.m.nfj, (long long)gi * (nj + 1), -1.0);
-        for (int v = 0; v < NV; ++v) a.qbar[v * N + c] = a.qbar[v * N + c] + add[v];
+        for (int v = 0; v < NV; ++v) {
+            const double qb = a.qbar[v * N + c] + add[v];
+            if (!a.qacc) a.qbar[v * N + c] = qb;
+            else a.qacc[v * N + c] = a.qacc_add ? a.qacc[v * N + c] + qb : qb;
+        }
     }
 };
 
@@ -551,7 +606,15 @@ template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k
 template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a)
 {
     const long long f = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (f < a.nfaces) AdjKernels<LAW, BASIS, FLUX>::adj_face(a, f);
+    if (f < a.nfaces) AdjKernels<LAW, BASIS, FLUX>::adj_face_interior(a, f);
+}
+// one thread per direction of each of the 2 nj + 2 ni boundary-face positions
+template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(64) k_adj_bface(AdjArgs a)
+{
+    using K = AdjKernels<LAW, BASIS, FLUX>;
+    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
+    const long long b = t / K::NDIR;
+    if (b < 2LL * (a.m.ni + a.m.nj)) K::adj_face_boundary(a, K::boundary_face(a.m, b), (int)(t % K::NDIR));
 }
 template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a)
 {

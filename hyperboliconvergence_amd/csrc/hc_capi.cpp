@@ -244,9 +244,10 @@ void release_buffers(hc_context* ctx)
     for (int k = 0; k < NTILECFG; ++k) {
         free_dev(ctx->rec64[k]);
         free_dev(ctx->rec32[k]);
-        free_dev(ctx->tile_case[k]);
+        free_dev(ctx->tiles[k]);
         ctx->ntiles[k] = 0;
     }
+    ctx->ngeo = 0;
     free_dev(ctx->chunks);
     ctx->nchunks = 0;
     free_dev(ctx->blk);
@@ -327,18 +328,18 @@ template <class RF> void fill_rec(const hc_context* ctx, const CaseHost& c, int 
     r.m.per_j = c.cs.bc[2] == HC_BC_PERIODIC;
     r.m.nfi = c.nfi;
     r.m.nfj = c.nfj;
-    r.m.cvol = ctx->cvol + c.cell_off;
-    r.m.ccx = ctx->ccx + c.cell_off;
-    r.m.ccy = ctx->ccy + c.cell_off;
+    r.m.cvol = ctx->cvol + c.geo_cell_off;
+    r.m.ccx = ctx->ccx + c.geo_cell_off;
+    r.m.ccy = ctx->ccy + c.geo_cell_off;
     if constexpr (sizeof(RF) == 8) {
-        r.m.fi = ctx->fi + NSLOT * c.fi_off;
-        r.m.fj = ctx->fj + NSLOT * c.fj_off;
+        r.m.fi = ctx->fi + NSLOT * c.geo_fi_off;
+        r.m.fj = ctx->fj + NSLOT * c.geo_fj_off;
     } else {
-        r.m.fi = ctx->fi32 + NSLOT * c.fi_off;
-        r.m.fj = ctx->fj32 + NSLOT * c.fj_off;
+        r.m.fi = ctx->fi32 + NSLOT * c.geo_fi_off;
+        r.m.fj = ctx->fj32 + NSLOT * c.geo_fj_off;
     }
-    r.m.fi64 = ctx->fi + NSLOT * c.fi_off;
-    r.m.fj64 = ctx->fj + NSLOT * c.fj_off;
+    r.m.fi64 = ctx->fi + NSLOT * c.geo_fi_off;
+    r.m.fj64 = ctx->fj + NSLOT * c.geo_fj_off;
     for (int d = 0; d < 2; ++d) {
         r.m.period_i[d] = c.cs.period_i[d];
         r.m.period_j[d] = c.cs.period_j[d];
@@ -351,7 +352,71 @@ template <class RF> void fill_rec(const hc_context* ctx, const CaseHost& c, int 
     r.cell_off = c.cell_off;
     r.bref_off = c.bref_off;
     r.tiles_j = c.tiles_j[k];
-    r.tile0 = c.tile0[k];
+}
+
+// bitwise equality of two cases' device geometry (cells and face slots)
+bool same_geometry(const GeomOutput& a, const GeomOutput& b)
+{
+    auto eq = [](const std::vector<double>& x, const std::vector<double>& y) {
+        return x.size() == y.size() && (x.empty() || std::memcmp(x.data(), y.data(), x.size() * sizeof(double)) == 0);
+    };
+    return eq(a.cvol, b.cvol) && eq(a.ccx, b.ccx) && eq(a.ccy, b.ccy) && eq(a.fi, b.fi) && eq(a.fj, b.fj);
+}
+
+// Case records and tile tables of every tile shape.  The table of a shape holds the tiles of one
+// step first: case by case, or with interleaving, every group of cases sharing a geometry block
+// tile by tile (tile lt of each case of the group, then tile lt + 1 ...), so that the work-groups
+// resident on an XCD together read the same face slots.  Each case's tiles follow in case order
+// (the part a single-case launch covers, from tile0).
+int build_tiles(hc_context* ctx)
+{
+    const TileShape* shapes = tile_shapes(ctx->precision == HC_PRECISION_MIXED, ctx->cs.law, ctx->cs.flux);
+    const int nc = (int)ctx->cases.size();
+    for (int k = 0; k < NTILECFG; ++k) {
+        long long nt = 0;
+        for (auto& c : ctx->cases) {
+            const int ti = (c.ni + shapes[k].ti - 1) / shapes[k].ti;
+            const int tj = (c.nj + shapes[k].tj - 1) / shapes[k].tj;
+            c.tiles_j[k] = tj;
+            c.ntiles[k] = ti * tj;
+            nt += (long long)ti * tj;
+        }
+        if (2 * nt > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "too many tiles");
+        std::vector<TileRef> tab;
+        tab.reserve((size_t)(2 * nt));
+        if (ctx->interleave) {
+            for (int g = 0; g < nc; ++g) {
+                if (ctx->cases[g].geo != g) continue;   // groups in the order of their first case
+                for (int lt = 0; lt < ctx->cases[g].ntiles[k]; ++lt)
+                    for (int ci = g; ci < nc; ++ci)
+                        if (ctx->cases[ci].geo == g) tab.push_back(TileRef{ci, lt});
+            }
+        } else {
+            for (int ci = 0; ci < nc; ++ci)
+                for (int lt = 0; lt < ctx->cases[ci].ntiles[k]; ++lt) tab.push_back(TileRef{ci, lt});
+        }
+        for (int ci = 0; ci < nc; ++ci) {
+            ctx->cases[ci].tile0[k] = (int)tab.size();
+            for (int lt = 0; lt < ctx->cases[ci].ntiles[k]; ++lt) tab.push_back(TileRef{ci, lt});
+        }
+        std::vector<CaseRec<double>> r64(nc);
+        std::vector<CaseRec<float>> r32(nc);
+        for (int ci = 0; ci < nc; ++ci) {
+            fill_rec(ctx, ctx->cases[ci], k, r64[ci]);
+            fill_rec(ctx, ctx->cases[ci], k, r32[ci]);
+        }
+        free_dev(ctx->rec64[k]);
+        free_dev(ctx->rec32[k]);
+        free_dev(ctx->tiles[k]);
+        HC_HIP(hipMalloc(&ctx->rec64[k], nc * sizeof(CaseRec<double>)));
+        HC_HIP(hipMalloc(&ctx->rec32[k], nc * sizeof(CaseRec<float>)));
+        HC_HIP(hipMalloc(&ctx->tiles[k], tab.size() * sizeof(TileRef)));
+        HC_HIP(hipMemcpy(ctx->rec64[k], r64.data(), nc * sizeof(CaseRec<double>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->rec32[k], r32.data(), nc * sizeof(CaseRec<float>), hipMemcpyHostToDevice));
+        HC_HIP(hipMemcpy(ctx->tiles[k], tab.data(), tab.size() * sizeof(TileRef), hipMemcpyHostToDevice));
+        ctx->ntiles[k] = (int)nt;
+    }
+    return HC_OK;
 }
 
 // device layout of all cases: packed arrays, case records, tile tables, update chunks
@@ -365,58 +430,74 @@ int build_layout(hc_context* ctx)
     const int nv = ctx->nv;
     const int nc = (int)ctx->cases.size();
     long long N = 0, nfi = 0, nfj = 0, br = 0;
-    long long nt[NTILECFG] = {};
-    const TileShape* shapes = tile_shapes(ctx->precision == HC_PRECISION_MIXED, ctx->cs.law, ctx->cs.flux);
-    for (auto& c : ctx->cases) {
+    long long gN = 0, gfi = 0, gfj = 0;   // packed geometry: one block per group of equal geometries
+    ctx->ngeo = 0;
+    for (int ci = 0; ci < nc; ++ci) {
+        CaseHost& c = ctx->cases[ci];
         c.cell_off = N;
-        c.fi_off = nfi;
-        c.fj_off = nfj;
         c.bref_off = br;
         N += c.N;
         nfi += c.nfi;
         nfj += c.nfj;
         br += c.brows;
-        for (int k = 0; k < NTILECFG; ++k) {
-            const int ti = (c.ni + shapes[k].ti - 1) / shapes[k].ti;
-            const int tj = (c.nj + shapes[k].tj - 1) / shapes[k].tj;
-            c.tiles_j[k] = tj;
-            c.ntiles[k] = ti * tj;
-            c.tile0[k] = (int)nt[k];
-            nt[k] += (long long)ti * tj;
+        c.geo = ci;
+        for (int g = 0; g < ci; ++g) {
+            const CaseHost& o = ctx->cases[g];
+            if (o.geo == g && o.ni == c.ni && o.nj == c.nj && same_geometry(o.g, c.g)) {
+                c.geo = g;
+                break;
+            }
+        }
+        if (c.geo == ci) {
+            c.geo_cell_off = gN;
+            c.geo_fi_off = gfi;
+            c.geo_fj_off = gfj;
+            gN += c.N;
+            gfi += c.nfi;
+            gfj += c.nfj;
+            ctx->ngeo++;
+        } else {
+            const CaseHost& o = ctx->cases[c.geo];
+            c.geo_cell_off = o.geo_cell_off;
+            c.geo_fi_off = o.geo_fi_off;
+            c.geo_fj_off = o.geo_fj_off;
         }
     }
-    for (int k = 0; k < NTILECFG; ++k)
-        if (nt[k] > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "too many tiles");
     ctx->Ntot = N;
     ctx->nfi_tot = nfi;
     ctx->nfj_tot = nfj;
     ctx->brows_tot = br;
-    // geometry
-    HC_HIP(hipMalloc(&ctx->cvol, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->ccx, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->ccy, N * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fi, NSLOT * nfi * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fj, NSLOT * nfj * sizeof(double)));
-    HC_HIP(hipMalloc(&ctx->fi32, NSLOT * nfi * sizeof(float)));
-    HC_HIP(hipMalloc(&ctx->fj32, NSLOT * nfj * sizeof(float)));
-    for (auto& c : ctx->cases) {
-        HC_HIP(hipMemcpy(ctx->cvol + c.cell_off, c.g.cvol.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->ccx + c.cell_off, c.g.ccx.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->ccy + c.cell_off, c.g.ccy.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->fi + NSLOT * c.fi_off, c.g.fi.data(), c.g.fi.size() * sizeof(double),
-                         hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->fj + NSLOT * c.fj_off, c.g.fj.data(), c.g.fj.size() * sizeof(double),
-                         hipMemcpyHostToDevice));
-        std::vector<float> f32;
-        fp32_slots(c.g.fi, c.nfi, f32);
-        HC_HIP(hipMemcpy(ctx->fi32 + NSLOT * c.fi_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
-        fp32_slots(c.g.fj, c.nfj, f32);
-        HC_HIP(hipMemcpy(ctx->fj32 + NSLOT * c.fj_off, f32.data(), f32.size() * sizeof(float), hipMemcpyHostToDevice));
+    // geometry, one block per group
+    HC_HIP(hipMalloc(&ctx->cvol, gN * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->ccx, gN * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->ccy, gN * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fi, NSLOT * gfi * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fj, NSLOT * gfj * sizeof(double)));
+    HC_HIP(hipMalloc(&ctx->fi32, NSLOT * gfi * sizeof(float)));
+    HC_HIP(hipMalloc(&ctx->fj32, NSLOT * gfj * sizeof(float)));
+    for (int ci = 0; ci < nc; ++ci) {
+        CaseHost& c = ctx->cases[ci];
+        if (c.geo == ci) {
+            HC_HIP(hipMemcpy(ctx->cvol + c.geo_cell_off, c.g.cvol.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+            HC_HIP(hipMemcpy(ctx->ccx + c.geo_cell_off, c.g.ccx.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+            HC_HIP(hipMemcpy(ctx->ccy + c.geo_cell_off, c.g.ccy.data(), c.N * sizeof(double), hipMemcpyHostToDevice));
+            HC_HIP(hipMemcpy(ctx->fi + NSLOT * c.geo_fi_off, c.g.fi.data(), c.g.fi.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+            HC_HIP(hipMemcpy(ctx->fj + NSLOT * c.geo_fj_off, c.g.fj.data(), c.g.fj.size() * sizeof(double),
+                             hipMemcpyHostToDevice));
+            std::vector<float> f32;
+            fp32_slots(c.g.fi, c.nfi, f32);
+            HC_HIP(hipMemcpy(ctx->fi32 + NSLOT * c.geo_fi_off, f32.data(), f32.size() * sizeof(float),
+                             hipMemcpyHostToDevice));
+            fp32_slots(c.g.fj, c.nfj, f32);
+            HC_HIP(hipMemcpy(ctx->fj32 + NSLOT * c.geo_fj_off, f32.data(), f32.size() * sizeof(float),
+                             hipMemcpyHostToDevice));
+        }
         const GeomInput gin{c.ni, c.nj, nullptr, {c.cs.bc[0], c.cs.bc[1], c.cs.bc[2], c.cs.bc[3]}, {0.0, 0.0}, {0.0, 0.0}};
         c.uniform = uniform_geometry(gin, c.g);
         c.X = std::move(c.g.X);
-        c.g = GeomOutput{};   // the device holds it now
     }
+    for (auto& c : ctx->cases) c.g = GeomOutput{};   // the device holds it now
     // state, accumulators, per-case slots
     for (auto& b : ctx->q) {
         HC_HIP(hipMalloc(&b, (size_t)nv * N * sizeof(double)));
@@ -438,23 +519,9 @@ int build_layout(hc_context* ctx)
         HC_HIP(hipMemcpy(ctx->dtslot, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     // case records and tile tables per tile shape
-    for (int k = 0; k < NTILECFG; ++k) {
-        std::vector<CaseRec<double>> r64(nc);
-        std::vector<CaseRec<float>> r32(nc);
-        std::vector<int> tc((size_t)nt[k]);
-        for (int ci = 0; ci < nc; ++ci) {
-            const CaseHost& c = ctx->cases[ci];
-            fill_rec(ctx, c, k, r64[ci]);
-            fill_rec(ctx, c, k, r32[ci]);
-            for (int t = 0; t < c.ntiles[k]; ++t) tc[(size_t)c.tile0[k] + t] = ci;
-        }
-        HC_HIP(hipMalloc(&ctx->rec64[k], nc * sizeof(CaseRec<double>)));
-        HC_HIP(hipMalloc(&ctx->rec32[k], nc * sizeof(CaseRec<float>)));
-        HC_HIP(hipMalloc(&ctx->tile_case[k], tc.size() * sizeof(int)));
-        HC_HIP(hipMemcpy(ctx->rec64[k], r64.data(), nc * sizeof(CaseRec<double>), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->rec32[k], r32.data(), nc * sizeof(CaseRec<float>), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->tile_case[k], tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice));
-        ctx->ntiles[k] = (int)nt[k];
+    {
+        const int r = build_tiles(ctx);
+        if (r) return r;
     }
     // 256-cell chunks of the deferred global-dt update
     std::vector<UpdChunk> ch;
@@ -474,7 +541,7 @@ template <class RF> LaunchArgs<RF> launch_args(hc_context* ctx)
     LaunchArgs<RF> L{};
     if constexpr (sizeof(RF) == 8) L.cases = ctx->rec64[ctx->tile_cfg];
     else L.cases = ctx->rec32[ctx->tile_cfg];
-    L.tile_case = ctx->tile_case[ctx->tile_cfg];
+    L.tiles = ctx->tiles[ctx->tile_cfg];
     L.tile_base = 0;
     L.ntiles = ctx->ntiles[ctx->tile_cfg];
     L.nslots = ctx->nslots;
@@ -748,7 +815,7 @@ int norms_of_res(hc_context* ctx, int k, double* l2, double* linf)
 {
     const CaseHost& c = ctx->cases[k];
     hipLaunchKernelGGL(k_norm_partials, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream,
-                       ctx->res + (ctx->nv + 1) * c.cell_off, ctx->cvol + c.cell_off, c.N, ctx->nv, ctx->part);
+                       ctx->res + (ctx->nv + 1) * c.cell_off, ctx->cvol + c.geo_cell_off, c.N, ctx->nv, ctx->part);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(256), 0, ctx->stream, ctx->part, NORM_BLOCKS, c.N, ctx->nout);
     HC_HIP(hipGetLastError());
     double out[8];
@@ -979,36 +1046,8 @@ int hc_set_precision(hc_context* ctx, int precision)
     HC_HIP(hipSetDevice(ctx->device));
     HC_HIP(hipStreamSynchronize(ctx->stream));
     drop_graph(ctx);
-    const TileShape* shapes = tile_shapes(precision == HC_PRECISION_MIXED, ctx->cs.law, ctx->cs.flux);
-    const int nc = (int)ctx->cases.size();
-    long long nt[NTILECFG] = {};
-    for (auto& c : ctx->cases)
-        for (int k = 0; k < NTILECFG; ++k) {
-            const int ti = (c.ni + shapes[k].ti - 1) / shapes[k].ti;
-            const int tj = (c.nj + shapes[k].tj - 1) / shapes[k].tj;
-            c.tiles_j[k] = tj;
-            c.ntiles[k] = ti * tj;
-            c.tile0[k] = (int)nt[k];
-            nt[k] += (long long)ti * tj;
-        }
-    for (int k = 0; k < NTILECFG; ++k) {
-        if (nt[k] > (1LL << 31) - 1) return fail(ctx, HC_ERR_ARG, "too many tiles");
-        std::vector<CaseRec<double>> r64(nc);
-        std::vector<CaseRec<float>> r32(nc);
-        std::vector<int> tc((size_t)nt[k]);
-        for (int ci = 0; ci < nc; ++ci) {
-            const CaseHost& c = ctx->cases[ci];
-            fill_rec(ctx, c, k, r64[ci]);
-            fill_rec(ctx, c, k, r32[ci]);
-            for (int t = 0; t < c.ntiles[k]; ++t) tc[(size_t)c.tile0[k] + t] = ci;
-        }
-        free_dev(ctx->tile_case[k]);
-        HC_HIP(hipMalloc(&ctx->tile_case[k], tc.size() * sizeof(int)));
-        HC_HIP(hipMemcpy(ctx->rec64[k], r64.data(), nc * sizeof(CaseRec<double>), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->rec32[k], r32.data(), nc * sizeof(CaseRec<float>), hipMemcpyHostToDevice));
-        HC_HIP(hipMemcpy(ctx->tile_case[k], tc.data(), tc.size() * sizeof(int), hipMemcpyHostToDevice));
-        ctx->ntiles[k] = (int)nt[k];
-    }
+    const int r = build_tiles(ctx);
+    if (r) return r;
     ctx->tuned = false;
     ctx->tile_cfg = ctx->tile_mode >= 0 ? ctx->tile_mode : 0;
     return HC_OK;
@@ -1222,6 +1261,30 @@ int hc_get_uniform_geometry(hc_context* ctx, int* active)
 {
     if (!ctx || !active) return HC_ERR_ARG;
     *active = ctx->have_case && ctx->uni_ok && ctx->uni_enable ? 1 : 0;
+    return HC_OK;
+}
+
+int hc_set_tile_order(hc_context* ctx, int interleave)
+{
+    if (!ctx) return HC_ERR_ARG;
+    if (ctx->building) return fail(ctx, HC_ERR_STATE, "batch not finished (hc_batch_end)");
+    if (ctx->interleave == (interleave != 0)) return HC_OK;
+    ctx->interleave = interleave != 0;
+    if (!ctx->have_case) return HC_OK;
+    HC_HIP(hipSetDevice(ctx->device));
+    HC_HIP(hipStreamSynchronize(ctx->stream));
+    drop_graph(ctx);
+    const int r = build_tiles(ctx);
+    if (r) return r;
+    ctx->tuned = false;
+    ctx->tile_cfg = ctx->tile_mode >= 0 ? ctx->tile_mode : 0;
+    return HC_OK;
+}
+
+int hc_get_geometry_blocks(hc_context* ctx, int* nblocks)
+{
+    if (!ctx || !nblocks) return HC_ERR_ARG;
+    *nblocks = ctx->have_case ? ctx->ngeo : 0;
     return HC_OK;
 }
 

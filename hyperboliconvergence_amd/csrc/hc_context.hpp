@@ -29,7 +29,11 @@ struct CaseHost {
     GeomOutput g;                  // host geometry until it is uploaded
     std::vector<double> X;         // LSQ spatial metric per cell {xx, xy, yy}, kept for hc_xmetrics
     long long N = 0, nfi = 0, nfj = 0;
-    long long cell_off = 0, fi_off = 0, fj_off = 0, bref_off = 0;
+    long long cell_off = 0, bref_off = 0;
+    // geometry block: the first case whose geometry is bitwise equal to this one's, and the
+    // block's offsets into the packed geometry arrays (cells, i-faces, j-faces)
+    int geo = 0;
+    long long geo_cell_off = 0, geo_fi_off = 0, geo_fj_off = 0;
     int brows = 0;                 // boundary reference rows 2 nj + 2 ni
     bool have_state = false;
     bool uniform = false;          // uniform geometry (hc_geom.hpp uniform_geometry)
@@ -68,8 +72,10 @@ struct hc_context {
     unsigned int* status = nullptr;
     CaseRec<double>* rec64[NTILECFG] = {};
     CaseRec<float>* rec32[NTILECFG] = {};
-    int* tile_case[NTILECFG] = {};
-    int ntiles[NTILECFG] = {};
+    hc::TileRef* tiles[NTILECFG] = {};   // tile table per tile shape: step order, then each case's tiles
+    int ntiles[NTILECFG] = {};           // tiles of a step (the step-order part of the table)
+    bool interleave = true;              // step order interleaves cases that share geometry (hc_set_tile_order)
+    int ngeo = 0;                        // geometry blocks held on the device
     UpdChunk* chunks = nullptr;
     int nchunks = 0;
     // tile shape: fixed index, or -1 to time every shape on the first steps and keep the fastest

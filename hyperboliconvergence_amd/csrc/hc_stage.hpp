@@ -747,8 +747,9 @@ template <int LAW, int BASIS> struct Stage0Update {
 // ---------------------------------------------------------------------------------------
 // Batched launch: one launch per stage covers the tiles of every case of a context.  The
 // per-case arrays are packed one case after the other (state [nv][N_c] blocks, face slots
-// [NSLOT][nf_c] blocks); a case record holds the case's mesh view with its pointers already
-// offset, its species, boundary types and offsets into the packed per-cell arrays.
+// [NSLOT][nf_c] blocks; cases whose geometry is bitwise equal share one geometry block); a case
+// record holds the case's mesh view with its pointers already offset, its species, boundary types
+// and offsets into the packed per-cell arrays.
 template <class RF> struct CaseRec {
     MeshView<RF> m;
     Species<double> sp;
@@ -758,13 +759,21 @@ template <class RF> struct CaseRec {
     long long cell_off;   // first cell of the case in the packed per-cell arrays
     long long bref_off;   // first boundary-reference row of the case
     int tiles_j;          // tiles along j in the record's tile shape
-    int tile0;            // first tile of the case in the launch's tile space
+};
+
+// One entry of a context's tile table: the case and the tile within the case (row-major over the
+// case's TI x TJ tiles).  Cases that share their geometry are interleaved tile by tile in the
+// table, so the work-groups an XCD runs together read the same face slots (one HBM fetch, the
+// rest L2 hits); the table also holds every case's tiles contiguously for single-case launches.
+struct TileRef {
+    int ci;
+    int lt;
 };
 
 template <class RF> struct LaunchArgs {
     const CaseRec<RF>* cases;
-    const int* tile_case;   // case of every tile of the context's tile space
-    int tile_base;          // first tile of the launch in that space
+    const TileRef* tiles;   // the context's tile table
+    int tile_base;          // first entry of the launch in the table
     int ntiles;             // tiles of the launch
     int nslots;
     int vscale, pscale, limiter;
@@ -866,11 +875,10 @@ k_stage(LaunchArgs<RF> L)
     const int tid = threadIdx.x;
     // XCD-aware tile order: consecutive work-group ids are dealt round-robin to the
     // 8 XCDs, so give each XCD a contiguous band of tiles (shared halos stay in one L2)
-    const int t = L.tile_base + xcd_tile(blockIdx.x, L.ntiles);
-    const int ci = L.tile_case[t];
+    const TileRef tr = L.tiles[L.tile_base + xcd_tile(blockIdx.x, L.ntiles)];
+    const int ci = tr.ci, lt = tr.lt;
     const CaseRec<RF>& c = L.cases[ci];
     const StageArgs<RF> a = case_args<RF, T::NV>(L, c, ci);
-    const int lt = t - c.tile0;
     const int i0 = (lt / c.tiles_j) * TI, j0 = (lt % c.tiles_j) * TJ;
     unsigned int flag = 0;
     T::load(a, s, i0, j0, tid);

@@ -238,6 +238,18 @@ int hc_set_uniform_geometry(hc_context* ctx, int enable);
 int hc_get_uniform_geometry(hc_context* ctx, int* active);
 
 /*
+ * Shared geometry and tile order of a batch: cases whose cell and face geometry is bitwise equal
+ * (the same mesh and boundary kinds, as in a sweep of flow conditions over a few meshes) share one
+ * device copy of it.  interleave = 1 (default) orders a step's work-groups so that the cases of
+ * such a group run tile by tile together (their face slots are fetched once and then read from
+ * L2); 0 runs the cases one after the other.  Results are bit-identical either way.
+ * hc_get_geometry_blocks returns the number of geometry copies held on the device.  No
+ * counterpart in the reference (one mesh per integrate() call there).
+ */
+int hc_set_tile_order(hc_context* ctx, int interleave);
+int hc_get_geometry_blocks(hc_context* ctx, int* nblocks);
+
+/*
  * Derivatives (single-case context, FP64; Euler with every flux in the conserved basis, AUSM+up
  * also in the primitive basis).  No counterpart in the reference.
  * Parameters: the state q, minf of the species (AUSM+up / SLAU / RoeUnprecWS scaling) and the boundary

@@ -119,6 +119,37 @@ def test_batch_equals_single_case_contexts(precision):
         assert np.array_equal(qb[k], q), k
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_shared_geometry_and_interleaved_tile_order_keep_every_bit(precision):
+    """Cases on the same mesh share one device copy of the geometry (two meshes, one of them also
+    ragged for every tile shape, with different Mach numbers and angles); interleaving their tiles
+    in a step changes nothing: every tile shape, both orders and single-case contexts agree bit for
+    bit, eager and in graphs."""
+    import hyperboliconvergence_amd as hca
+    cfgs = [_cyl(40, 21, 0.3, 0.0, perturb=0.01, seed=1), _cyl(64, 32, 0.55, 4.0, perturb=0.01, seed=2),
+            _cyl(40, 21, 0.45, -3.0, perturb=0.01, seed=3), _cyl(64, 32, 0.25, 2.0, perturb=0.01, seed=4),
+            _cyl(40, 21, 0.7, 1.5, perturb=0.01, seed=5)]
+    ref = []
+    for cfg in cfgs:
+        with _single(cfg, precision) as ctx:
+            r = ctx.residual()
+            ctx.step(7)
+            ref.append((r, ctx.get_state()))
+    for tiles in range(hca.NTILECFG):
+        for order in (1, 0, 1):   # switching the order back and forth rebuilds the table
+            with _batch(cfgs, precision) as b:
+                assert b.geometry_blocks() == 2
+                b.set_tile_order(order)
+                b.set_tiles(tiles)
+                b.set_graph_steps(3 if order else 0)
+                res = [b.case_residual(k) for k in range(len(cfgs))]
+                b.step(7)
+                assert b.status() == 0
+                for k in range(len(cfgs)):
+                    assert np.array_equal(res[k], ref[k][0]), (tiles, order, k)
+                    assert np.array_equal(b.get_case_state(k), ref[k][1]), (tiles, order, k)
+
+
 def test_batch_fp64_matches_oracle_per_case():
     cfgs = _mixed_batch_cfgs()[:3]
     with _batch(cfgs) as b:
