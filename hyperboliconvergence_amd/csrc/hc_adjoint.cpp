@@ -24,13 +24,14 @@ struct AdjState {
     int nv = 0, ns = 0;
     // residual-level work arrays
     double *Q = nullptr, *Qd = nullptr, *F = nullptr, *Fd = nullptr, *res = nullptr, *resd = nullptr;
-    double *fout = nullptr, *minf_face = nullptr, *brefbar = nullptr, *qbar = nullptr, *Qbar = nullptr;
+    double *minf_face = nullptr, *brefbar = nullptr, *qbar = nullptr, *Qbar = nullptr;
     double *ybar_in = nullptr, *qd_in = nullptr, *brefd_in = nullptr, *part = nullptr, *scal = nullptr;
     // trajectory arrays
     std::vector<double*> ckpt;       // checkpoints, one state each
-    std::vector<double*> seg;        // states of one segment
-    std::vector<double*> stage;      // stage states q_1 .. q_{ns-1}
-    std::vector<double*> a, yb;      // adjoints of the stage outputs / of the stage residuals
+    std::vector<double*> seg;        // stage states of one segment, C x ns
+    std::vector<double*> stage;      // stage states q_1 .. q_{ns-1} (tangent steps)
+    std::vector<double*> a;          // the second step-boundary adjoint of the backward sweep
+    std::vector<double*> yb;         // adjoints of the stage residuals
     std::vector<double*> rd;         // tangent residuals of the stages (AoS [N][nv])
     std::vector<double*> qdot;       // stage tangents q_dot_0 .. q_dot_{ns-1}
     double *qbn = nullptr, *brefbar_tot = nullptr, *qtmp = nullptr;
@@ -71,28 +72,11 @@ __global__ void __launch_bounds__(256) k_sum_final(const double* part, int n, do
     if (threadIdx.x == 0) out[slot] = accumulate ? out[slot] + s[0] : s[0];
 }
 
-// Runge-Kutta adjoint of one stage, given x = the adjoint of the stage's output state:
-//   lam (+)= x  and  yb_k (+)= (coef_k / V_c) x  for the residuals k the stage combines,
-// each target assigned on its first contribution of the step and added to afterwards
-struct RkAdj {
-    double* yb[6];
-    double coef[6];
-    int add[6];
-    int nk;
-    int lam_add;
-};
-__global__ void __launch_bounds__(256) k_rk_adjoint(double* lam, const double* x, RkAdj t, const double* vol,
-                                                    long long N, int nv)
+// Runge-Kutta adjoint of one stage (rk_deliver) over every unknown
+__global__ void __launch_bounds__(256) k_rk_adjoint(const double* x, RkDeliver t, const double* vol, long long N, int nv)
 {
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (i >= N * nv) return;
-    const double xi = x[i];
-    lam[i] = t.lam_add ? lam[i] + xi : xi;
-    const double v = vol[i % N];
-    for (int k = 0; k < t.nk; ++k) {
-        const double y = (t.coef[k] / v) * xi;
-        t.yb[k][i] = t.add[k] ? t.yb[k][i] + y : y;
-    }
+    if (i < N * nv) rk_deliver(t, i, x[i], vol[i % N]);
 }
 
 // stage tangent: out[v][c] = base[v][c] + (beta dt / V_c) sum_k coef[k] rd_k[c][v]
@@ -160,7 +144,7 @@ int ensure(hc_context* ctx)
     int r = 0;
     if ((r = alloc(ctx, s.Q, 2 * nv * N)) || (r = alloc(ctx, s.Qd, 2 * nv * N)) || (r = alloc(ctx, s.F, (nv + 1) * nF)) ||
         (r = alloc(ctx, s.Fd, nv * nF)) || (r = alloc(ctx, s.res, (nv + 1) * N)) || (r = alloc(ctx, s.resd, nv * N)) ||
-        (r = alloc(ctx, s.fout, 6 * nv * nF)) || (r = alloc(ctx, s.minf_face, nF)) ||
+        (r = alloc(ctx, s.minf_face, nF)) ||
         (r = alloc(ctx, s.brefbar, (long long)c.brows * nv)) || (r = alloc(ctx, s.qbar, nv * N)) ||
         (r = alloc(ctx, s.Qbar, 2 * nv * N)) || (r = alloc(ctx, s.ybar_in, nv * N)) || (r = alloc(ctx, s.qd_in, nv * N)) ||
         (r = alloc(ctx, s.brefd_in, (long long)c.brows * nv)) || (r = alloc(ctx, s.part, SUM_BLOCKS)) ||
@@ -203,7 +187,6 @@ AdjArgs base_adj(hc_context* ctx)
     a.Fd = s.Fd;
     a.res = s.res;
     a.resd = s.resd;
-    a.fout = s.fout;
     a.minf_face = s.minf_face;
     a.brefbar = s.brefbar;
     a.qbar = s.qbar;
@@ -256,10 +239,10 @@ int download_soa(hc_context* ctx, double* aos, const double* src, long long N, i
     return HC_OK;
 }
 
-// adjoint residual at state q (device SoA) with ybar (device SoA): qbar delivered into dst (assigned,
-// or added when add is set); minf_bar added to s.scal[0]; the boundary-reference adjoint added to
-// s.brefbar_tot
-int adjoint_at(hc_context* ctx, const double* q, const double* ybar, double* dst, bool add)
+// adjoint residual at state q (device SoA) with ybar (device SoA): qbar delivered as the Runge-Kutta
+// adjoint rk when given, else into dst (assigned, or added when add is set); the face minf adjoints
+// added to s.minf_face, the boundary-reference adjoint to s.brefbar_tot
+int adjoint_at(hc_context* ctx, const double* q, const double* ybar, double* dst, bool add, const RkDeliver* rk)
 {
     AdjState& s = *ctx->adj;
     AdjArgs a = base_adj(ctx);
@@ -269,9 +252,20 @@ int adjoint_at(hc_context* ctx, const double* q, const double* ybar, double* dst
     a.bref_acc = 1;
     a.qacc = dst;
     a.qacc_add = add ? 1 : 0;
+    if (rk) {
+        a.rk_on = 1;
+        a.rk = *rk;
+    }
     HC_HIP(run_adjoint(ctx, a));
+    return HC_OK;
+}
+
+// minf_bar = the sum of the per-face minf adjoints, into s.scal[0]
+int minf_total(hc_context* ctx)
+{
+    AdjState& s = *ctx->adj;
     hipLaunchKernelGGL(k_sum_partials, dim3(SUM_BLOCKS), dim3(256), 0, ctx->stream, s.minf_face, s.nfaces, s.part);
-    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, ctx->stream, s.part, SUM_BLOCKS, s.scal, 0, 1);
+    hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(256), 0, ctx->stream, s.part, SUM_BLOCKS, s.scal, 0, 0);
     HC_HIP(hipGetLastError());
     return HC_OK;
 }
@@ -316,7 +310,7 @@ void release_adjoint(hc_context* ctx)
 {
     if (!ctx->adj) return;
     AdjState& s = *ctx->adj;
-    for (double** p : {&s.Q, &s.Qd, &s.F, &s.Fd, &s.res, &s.resd, &s.fout, &s.minf_face, &s.brefbar, &s.qbar, &s.Qbar,
+    for (double** p : {&s.Q, &s.Qd, &s.F, &s.Fd, &s.res, &s.resd, &s.minf_face, &s.brefbar, &s.qbar, &s.Qbar,
                        &s.ybar_in, &s.qd_in, &s.brefd_in, &s.part, &s.scal, &s.qbn, &s.brefbar_tot, &s.qtmp})
         free_d(*p);
     for (auto* v : {&s.ckpt, &s.seg, &s.stage, &s.a, &s.yb, &s.rd, &s.qdot})
@@ -364,8 +358,9 @@ int hc_adjoint_residual(hc_context* ctx, const double* ybar, double* qbar, doubl
     if ((r = upload_soa(ctx, s.ybar_in, ybar, s.N, s.nv))) return r;
     const long long nb = (long long)c.brows * s.nv;
     HC_HIP(hipMemsetAsync(s.brefbar_tot, 0, nb * sizeof(double), ctx->stream));
-    HC_HIP(hipMemsetAsync(s.scal, 0, 4 * sizeof(double), ctx->stream));
-    if ((r = adjoint_at(ctx, ctx->q[ctx->cur], s.ybar_in, s.qbar, false))) return r;
+    HC_HIP(hipMemsetAsync(s.minf_face, 0, s.nfaces * sizeof(double), ctx->stream));
+    if ((r = adjoint_at(ctx, ctx->q[ctx->cur], s.ybar_in, s.qbar, false, nullptr))) return r;
+    if ((r = minf_total(ctx))) return r;
     if ((r = download_soa(ctx, qbar, s.qbar, s.N, s.nv))) return r;
     HC_HIP(hipMemcpy(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToHost));
     HC_HIP(hipMemcpy(minf_bar, s.scal, sizeof(double), hipMemcpyDeviceToHost));
@@ -412,11 +407,11 @@ static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* q
     const double dt = ctx->cs.cfl;
     // every stage state of a segment is kept (C steps x ns stages), so the reverse sweep recomputes
     // each step once, from the segment's checkpoint
-    if ((r = grow(ctx, s.seg, (size_t)s.C * ns, len)) || (r = grow(ctx, s.a, ns + 1, len)) || (r = grow(ctx, s.yb, ns, len)))
+    if ((r = grow(ctx, s.seg, (size_t)s.C * ns, len)) || (r = grow(ctx, s.a, 1, len)) || (r = grow(ctx, s.yb, ns, len)))
         return r;
     // lam holds the adjoint of the state at the current step boundary; nxt receives the one before
     double* lam = s.qbn;
-    double* nxt = s.a[ns];
+    double* nxt = s.a[0];
     if (dev) {
         HC_HIP(hipStreamSynchronize(ctx->stream));
         if ((r = aos_to_soa_device(ctx, lam, qbar_final, N, nv))) return r;
@@ -424,7 +419,7 @@ static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* q
         return r;
     }
     HC_HIP(hipMemsetAsync(s.brefbar_tot, 0, nb * sizeof(double), ctx->stream));
-    HC_HIP(hipMemsetAsync(s.scal, 0, 4 * sizeof(double), ctx->stream));
+    HC_HIP(hipMemsetAsync(s.minf_face, 0, s.nfaces * sizeof(double), ctx->stream));
     const int nseg = (s.K + s.C - 1) / s.C;
     for (int g = nseg - 1; g >= 0; --g) {
         const int n0 = g * s.C;
@@ -443,37 +438,47 @@ static int adjoint_backward(hc_context* ctx, const double* qbar_final, double* q
             }
         }
         for (int t = n - 1; t >= 0; --t) {
-            // a[st]: adjoint of stage st's output (the last stage's output is the step's end state)
-            double* a[6];
-            for (int st = 0; st + 1 < ns; ++st) a[st] = s.a[st];
-            a[ns - 1] = lam;
+            // Runge-Kutta adjoint of stage st: its output's adjoint goes to nxt (the step's start
+            // state enters every stage) and, scaled, to the residual adjoints yb[k] it combines
             bool yb_set[6] = {false, false, false, false, false, false};
-            for (int st = ns - 1; st >= 0; --st) {
-                RkAdj t_adj{};
-                t_adj.lam_add = st < ns - 1 ? 1 : 0;
+            auto program = [&](int st) {
+                RkDeliver d{};
+                d.lam = nxt;
+                d.lam_add = st < ns - 1 ? 1 : 0;
                 for (int k = 0; k <= st; ++k) {
                     const double alpha = ctx->cs.alpha[st][k];
                     if (alpha == 0.0) continue;
-                    t_adj.yb[t_adj.nk] = s.yb[k];
-                    t_adj.coef[t_adj.nk] = alpha * (ctx->cs.beta[st] * dt);
-                    t_adj.add[t_adj.nk] = yb_set[k] ? 1 : 0;
-                    t_adj.nk++;
+                    d.yb[d.nk] = s.yb[k];
+                    d.coef[d.nk] = alpha * (ctx->cs.beta[st] * dt);
+                    d.add[d.nk] = yb_set[k] ? 1 : 0;
+                    d.nk++;
                     yb_set[k] = true;
                 }
-                hipLaunchKernelGGL(k_rk_adjoint, dim3(blocks(len)), dim3(256), 0, ctx->stream, nxt, a[st], t_adj,
-                                   ctx->cvol, N, nv);
-                HC_HIP(hipGetLastError());
+                return d;
+            };
+            // the last stage's output is the step's end state, whose adjoint is lam
+            const RkDeliver last = program(ns - 1);
+            hipLaunchKernelGGL(k_rk_adjoint, dim3(blocks(len)), dim3(256), 0, ctx->stream, lam, last, ctx->cvol, N, nv);
+            HC_HIP(hipGetLastError());
+            for (int st = ns - 1; st >= 0; --st) {
                 if (!yb_set[st]) {   // no stage combines R_st: its adjoint weight is zero
                     HC_HIP(hipMemsetAsync(s.yb[st], 0, len * sizeof(double), ctx->stream));
                     yb_set[st] = true;
                 }
-                // the adjoint residual of stage st feeds the previous stage's output, or the step's start
-                if ((r = adjoint_at(ctx, s.seg[(size_t)t * ns + st], s.yb[st], st > 0 ? a[st - 1] : nxt, st == 0)))
-                    return r;
+                // the adjoint residual of stage st is the adjoint of stage st - 1's output, delivered
+                // straight into that stage's Runge-Kutta adjoint; stage 0's feeds the step's start
+                if (st > 0) {
+                    const RkDeliver prev = program(st - 1);
+                    r = adjoint_at(ctx, s.seg[(size_t)t * ns + st], s.yb[st], nullptr, false, &prev);
+                } else {
+                    r = adjoint_at(ctx, s.seg[(size_t)t * ns], s.yb[0], nxt, true, nullptr);
+                }
+                if (r) return r;
             }
             std::swap(lam, nxt);
         }
     }
+    if ((r = minf_total(ctx))) return r;
     if (dev) {
         if ((r = soa_to_aos_device(ctx, qbar_initial, lam, N, nv))) return r;
         HC_HIP(hipMemcpyAsync(bref_bar, s.brefbar_tot, nb * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));

@@ -13,13 +13,13 @@
 //             Per face, the transposed flux Jacobian with respect to the reconstructed
 //             left/right states (and minf, or the boundary reference state) is applied to
 //             the face's adjoint weight: AUSM+up on interior faces by its hand-written
-//             reverse sweep (hc_flux_vjp.hpp), the other fluxes and the boundary faces
-//             from Dual<9> numbers (boundary faces in a kernel of their own, so that the
-//             interior kernel does not hold the Dual<9> registers); the MUSCL reconstruction (limiter
-//             derivatives from Dual<2>, the LSQ-biased slopes, which are linear in the
-//             LSQ solution metric Q) and the metric Q = sum (q_R - q_L) dx are then
-//             transposed by hand: faces write their contributions, cells gather them in
-//             a fixed order (deterministic, no atomics).
+//             reverse sweep (hc_flux_vjp.hpp), the other fluxes from Dual<9> numbers, the
+//             boundary faces one direction per lane from Dual<1> numbers in a kernel of
+//             their own; the MUSCL reconstruction (limiter derivatives from Dual<2>, the
+//             LSQ-biased slopes, which are linear in the LSQ solution metric Q) and the
+//             metric Q = sum (q_R - q_L) dx are then transposed by hand.  A work-group
+//             computes the faces of a tile of cells into LDS and its cells gather them in
+//             a fixed order (deterministic, no atomics; AdjTile).
 //
 // Faces are numbered as the face-geometry arrays: i-faces l*nj + j (node line l, row j),
 // then j-faces nfi + i*(nj+1) + l.  On a periodic direction the last node line is the
@@ -33,6 +33,27 @@
 #include "hc_stage.hpp"
 
 namespace hc {
+
+// Runge-Kutta adjoint of one stage, given x = the adjoint of the stage's output state:
+//   lam (+)= x  and  yb_k (+)= (coef_k / V_c) x  for the residuals k the stage combines,
+// each target assigned on its first contribution of the step and added to afterwards
+struct RkDeliver {
+    double* lam;
+    double* yb[6];
+    double coef[6];
+    int add[6];
+    int nk;
+    int lam_add;
+};
+
+HC_HD void rk_deliver(const RkDeliver& t, long long i, double x, double vol)
+{
+    t.lam[i] = t.lam_add ? t.lam[i] + x : x;
+    for (int k = 0; k < t.nk; ++k) {
+        const double y = (t.coef[k] / vol) * x;
+        t.yb[k][i] = t.add[k] ? t.yb[k][i] + y : y;
+    }
+}
 
 struct AdjArgs {
     MeshView<double> m;
@@ -52,14 +73,17 @@ struct AdjArgs {
     double* res;           // residual [N][nv+1] (layout of hc_residual)
     double* resd;          // its tangent [N][nv]
     const double* ybar;    // adjoint of the residual [nv][N]
-    double* fout;          // per-face adjoint contributions [6 nv][nfaces]
-    double* minf_face;     // per-face minf adjoint [nfaces]
+    double* minf_face;     // per-face minf adjoint [nfaces], accumulated over calls (zeroed by the caller)
     double* brefbar;       // [rows][nv] (rows without a face stay as they are)
     double* qbar;          // [nv][N]
     double* Qbar;          // [nv][2][N]
     int bref_acc;          // 1: boundary faces add their reference-state adjoint to brefbar
-    double* qacc;          // if set, adj_qedges delivers qbar into it instead of qbar:
-    int qacc_add;          //   qacc = qbar (0) or qacc = qacc + qbar (1)
+    // adj_qedges delivers qbar: with rk_on as the Runge-Kutta adjoint rk of the previous stage,
+    // else into qacc if set (qacc = qbar, or qacc = qacc + qbar with qacc_add), else into qbar
+    int rk_on;
+    RkDeliver rk;
+    double* qacc;
+    int qacc_add;
 };
 
 // geometry and cells of one face
@@ -330,60 +354,32 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
         a.res[c * (NV + 1) + NV] = lam;
     }
 
-    // ---- adjoint: metric of q (values only) ----
-    HC_HD static void adj_qmetric(const AdjArgs& a, long long c)
-    {
-        const int gi = (int)(c / a.m.nj), gj = (int)(c % a.m.nj);
-        double acc[NV][2];
-        qmetric_cell<double, NV>(a.m, a.q, nullptr, gi, gj, acc);
-        for (int v = 0; v < NV; ++v)
-            for (int d = 0; d < 2; ++d) a.Q[(v * 2 + d) * a.N + c] = acc[v][d];
-    }
-
     // ---- adjoint: transpose of face f's flux and reconstruction ----
-    // fout rows: [0, NV) to the left cell, [NV, 2NV) to the right cell, [2NV, 4NV) Qbar of the
-    // left cell (v*2+d), [4NV, 6NV) Qbar of the right cell.  A boundary face writes its interior
-    // cell's contribution on the side that cell occupies (right of node line 0, left of the last).
-    HC_HD static void adj_face(const AdjArgs& a, long long f)
-    {
-        if (face_ref(a.m, f).boundary)
-            for (int k = 0; k < NDIR; ++k) adj_face_boundary(a, f, k);
-        else
-            adj_face_interior(a, f);
-    }
+    // Contribution rows of a face: [0, NV) to the left cell, [NV, 2NV) to the right cell,
+    // [2NV, 4NV) Qbar of the left cell (v*2+d), [4NV, 6NV) Qbar of the right cell.  A boundary face
+    // gives its interior cell's contribution on the side that cell occupies (right of node line 0,
+    // left of the last).
 
-    // boundary-face index b in [0, 2 nj + 2 ni) -> face index (i-faces of node lines 0 and ni, then
-    // j-faces of node lines 0 and nj), whatever the directions' periodicity
-    HC_HD static long long boundary_face(const MeshView<double>& m, long long b)
-    {
-        const long long ni = m.ni, nj = m.nj;
-        if (b < nj) return b;
-        if (b < 2 * nj) return ni * nj + (b - nj);
-        if (b < 2 * nj + ni) return m.nfi + (b - 2 * nj) * (nj + 1);
-        return m.nfi + (b - 2 * nj - ni) * (nj + 1) + nj;
-    }
-
-    // Non-periodic boundary face, direction k of its transposed flux Jacobian: k < NV the interior
+    // Non-periodic boundary face f, direction k of its transposed flux Jacobian: k < NV the interior
     // state, k < 2 NV the boundary reference state, k = 2 NV minf; one Dual<1> evaluation per
     // direction (each direction's derivative has the bits it has in a wider dual evaluation).
-    // The directions of a face write disjoint outputs: k < NV its cell's row and a zero in the
-    // other side's row, NV <= k < 2 NV four of the Qbar rows (zeros) and the reference adjoint,
-    // k = 2 NV the face's minf adjoint.  Other faces are left alone.
+    // The directions of a face hand disjoint contribution rows to put(row, value): k < NV its
+    // cell's row and a zero in the other side's row, NV <= k < 2 NV four of the (zero) Qbar rows;
+    // they add the reference-state adjoint (k < 2 NV) to brefbar and the minf adjoint (k = 2 NV) to
+    // minf_face[f].
     static constexpr int NDIR = 2 * NV + 1;
-    HC_HD static void adj_face_boundary(const AdjArgs& a, long long f, int k)
+    template <class Put> HC_HD static void boundary_contrib(const AdjArgs& a, long long f, int k, Put&& put)
     {
         const FaceRef r = face_ref(a.m, f);
-        if (!r.valid || !r.boundary) return;
-        const long long nF = a.nfaces;
         const double* base = r.isi ? a.m.fi : a.m.fj;
         const long long nf = r.isi ? a.m.nfi : a.m.nfj;
         const long long N = a.N;
         const int type = bc_of(a, r.bid);
         const bool plus = (r.bid == 0 || r.bid == 2) || type == BC_WALL;
         const int side = (r.bid == 0 || r.bid == 2) ? NV : 0;   // interior cell right / left of the face
-        if (k < NV) a.fout[(NV - side + k) * nF + f] = 0.0;
+        if (k < NV) put(NV - side + k, 0.0);
         else if (k < 2 * NV)
-            for (int m = 0; m < 4; ++m) a.fout[(2 * NV + 4 * (k - NV) + m) * nF + f] = 0.0;
+            for (int m = 0; m < 4; ++m) put(2 * NV + 4 * (k - NV) + m, 0.0);
         const Species<D1> sp1 = dual_species<1>(a.sp, k == 2 * NV ? 0 : -1);
         D1 qi[NV], qf[NV];
         for (int v = 0; v < NV; ++v) {
@@ -398,36 +394,32 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
             g = g + Fb * F.f[v].d[0];
         }
         if (k < NV) {
-            a.fout[(side + k) * nF + f] = g;
+            put(side + k, g);
         } else if (k < 2 * NV) {
             double* dst = a.brefbar + r.row * NV + (k - NV);
             if (type != BC_WALL) *dst = a.bref_acc ? *dst + g : g;
         } else {
-            a.minf_face[f] = g;
+            a.minf_face[f] = a.minf_face[f] + g;
         }
     }
 
-    // interior or periodic face (boundary faces are left to adj_face_boundary)
-    HC_HD static void adj_face_interior(const AdjArgs& a, long long f)
+    // The 6 NV transposed contributions of a valid interior or periodic face, handed to put(row,
+    // value) in the contribution rows, and the face's minf adjoint in *minf_out.  The left / right
+    // cells' residual adjoints (yl, yr), states (ql, qr) and LSQ solution metrics (Ql, Qr) come in.
+    template <class Put>
+    HC_HD static void face_contrib(const AdjArgs& a, const FaceRef& r, const double* yl, const double* yr,
+                                   const double* ql, const double* qr, const double (*Ql)[2], const double (*Qr)[2],
+                                   double* minf_out, Put&& put)
     {
-        const FaceRef r = face_ref(a.m, f);
-        if (r.boundary) return;
-        const long long nF = a.nfaces;
-        for (int k = 0; k < 6 * NV; ++k) a.fout[k * nF + f] = 0.0;
-        a.minf_face[f] = 0.0;
-        if (!r.valid) return;
         const double* base = r.isi ? a.m.fi : a.m.fj;
         const long long nf = r.isi ? a.m.nfi : a.m.nfj;
-        const long long N = a.N;
         double Fb[NV];
-        for (int v = 0; v < NV; ++v) Fb[v] = a.ybar[v * N + r.cr] - a.ybar[v * N + r.cl];
+        for (int v = 0; v < NV; ++v) Fb[v] = yr[v] - yl[v];
         const double area = base[FS_AREA * nf + r.fidx];
         const double n0 = base[FS_N0 * nf + r.fidx];
         const double n1 = base[FS_N1 * nf + r.fidx];
-        double ql[NV], qr[NV], qlf[NV], qrf[NV];
+        double qlf[NV], qrf[NV];
         for (int v = 0; v < NV; ++v) {
-            ql[v] = a.q[v * N + r.cl];
-            qr[v] = a.q[v * N + r.cr];
             qlf[v] = ql[v];
             qrf[v] = qr[v];
         }
@@ -446,22 +438,18 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
                 fr[k] = base[(FS_FR0 + k) * nf + r.fidx];
             }
             for (int v = 0; v < NV; ++v) {
-                const double Ql0 = a.Q[(v * 2) * N + r.cl], Ql1 = a.Q[(v * 2 + 1) * N + r.cl];
-                const double Qr0 = a.Q[(v * 2) * N + r.cr], Qr1 = a.Q[(v * 2 + 1) * N + r.cr];
+                const double Ql0 = Ql[v][0], Ql1 = Ql[v][1];
+                const double Qr0 = Qr[v][0], Qr1 = Qr[v][1];
                 const double dq = qr[v] - ql[v];
                 const double dqdx = dq * s1;
                 const double c0 = dq * dx0;
                 const double c1 = dq * dx1;
                 const double gl = lsq_solve1(fl, Ql0 - c0, Ql1 - c1, n0, n1);
                 const double gr = lsq_solve1(fr, Qr0 - c0, Qr1 - c1, n0, n1);
-                const D2 sl = limit(a.limiter, make_dual<2>(dqdx, 0), make_dual<2>(gl, 1));
-                const D2 sr = limit(a.limiter, make_dual<2>(dqdx, 0), make_dual<2>(gr, 1));
-                Lla[v] = sl.d[0];
-                Llb[v] = sl.d[1];
-                Lra[v] = sr.d[0];
-                Lrb[v] = sr.d[1];
-                qlf[v] = ql[v] + sl.v * dl;
-                qrf[v] = qr[v] - sr.v * dr;
+                const double sl = limit_d(a.limiter, dqdx, gl, Lla[v], Llb[v]);
+                const double sr = limit_d(a.limiter, dqdx, gr, Lra[v], Lrb[v]);
+                qlf[v] = ql[v] + sl * dl;
+                qrf[v] = qr[v] - sr * dr;
             }
         }
         // transposed flux Jacobian (reconstructed states and minf) applied to Fb
@@ -492,12 +480,13 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
             }
             for (int v = 0; v < NV; ++v) mb = mb + Fb[v] * F.f[v].d[8];
         }
-        a.minf_face[f] = mb;
+        *minf_out = mb;
         if (!recon) {
             for (int v = 0; v < NV; ++v) {
-                a.fout[v * nF + f] = qlb[v];
-                a.fout[(NV + v) * nF + f] = qrb[v];
+                put(v, qlb[v]);
+                put(NV + v, qrb[v]);
             }
+            for (int k = 2 * NV; k < 6 * NV; ++k) put(k, 0.0);
             return;
         }
         // transpose of the reconstruction, variable by variable
@@ -513,36 +502,34 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
             const double u0r = grb * wr0, u1r = grb * wr1;
             const double c0b = -(u0l + u0r), c1b = -(u1l + u1r);
             const double dqb = dqdxb * s1 + c0b * dx0 + c1b * dx1;
-            a.fout[v * nF + f] = qlb[v] - dqb;
-            a.fout[(NV + v) * nF + f] = qrb[v] + dqb;
-            a.fout[(2 * NV + 2 * v) * nF + f] = u0l;
-            a.fout[(2 * NV + 2 * v + 1) * nF + f] = u1l;
-            a.fout[(4 * NV + 2 * v) * nF + f] = u0r;
-            a.fout[(4 * NV + 2 * v + 1) * nF + f] = u1r;
+            put(v, qlb[v] - dqb);
+            put(NV + v, qrb[v] + dqb);
+            put(2 * NV + 2 * v, u0l);
+            put(2 * NV + 2 * v + 1, u1l);
+            put(4 * NV + 2 * v, u0r);
+            put(4 * NV + 2 * v + 1, u1r);
         }
     }
 
     // ---- adjoint: gather of the face contributions per cell (qbar direct part, Qbar) ----
-    HC_HD static void adj_cell(const AdjArgs& a, long long c)
+    // cell c's direct adjoint and Qbar from its faces' contributions get(row, face) with face 0 / 1
+    // the left / right i-face and 2 / 3 the left / right j-face: the cell is right of its left faces
+    // and left of its right faces
+    template <class Get> HC_HD static void gather(const AdjArgs& a, long long c, Get&& get)
     {
-        const int i = (int)(c / a.m.nj), j = (int)(c % a.m.nj);
-        long long fl[4];
-        cell_faces(a.m, i, j, fl);
-        const long long nF = a.nfaces;
-        // the cell is right of its left faces and left of its right faces
         for (int v = 0; v < NV; ++v) {
             double s = 0.0;
-            s = s + a.fout[(NV + v) * nF + fl[0]];
-            s = s + a.fout[v * nF + fl[1]];
-            s = s + a.fout[(NV + v) * nF + fl[2]];
-            s = s + a.fout[v * nF + fl[3]];
+            s = s + get(NV + v, 0);
+            s = s + get(v, 1);
+            s = s + get(NV + v, 2);
+            s = s + get(v, 3);
             a.qbar[v * a.N + c] = s;
             for (int d = 0; d < 2; ++d) {
                 double t = 0.0;
-                t = t + a.fout[(4 * NV + 2 * v + d) * nF + fl[0]];
-                t = t + a.fout[(2 * NV + 2 * v + d) * nF + fl[1]];
-                t = t + a.fout[(4 * NV + 2 * v + d) * nF + fl[2]];
-                t = t + a.fout[(2 * NV + 2 * v + d) * nF + fl[3]];
+                t = t + get(4 * NV + 2 * v + d, 0);
+                t = t + get(2 * NV + 2 * v + d, 1);
+                t = t + get(4 * NV + 2 * v + d, 2);
+                t = t + get(2 * NV + 2 * v + d, 3);
                 a.Qbar[(v * 2 + d) * a.N + c] = t;
             }
         }
@@ -575,13 +562,197 @@ template <int LAW, int BASIS, int FLUX> struct AdjKernels {
         else if (a.m.per_j) edge(c, (long long)gi * nj, a.m.fj, a.m.nfj, (long long)gi * (nj + 1), -1.0);
         for (int v = 0; v < NV; ++v) {
             const double qb = a.qbar[v * N + c] + add[v];
-            if (!a.qacc) a.qbar[v * N + c] = qb;
+            if (a.rk_on) rk_deliver(a.rk, v * N + c, qb, a.m.cvol[c]);
+            else if (!a.qacc) a.qbar[v * N + c] = qb;
             else a.qacc[v * N + c] = a.qacc_add ? a.qacc[v * N + c] + qb : qb;
         }
     }
 };
 
+// Adjoint faces and cell gather of one TI x TJ tile of cells in one work-group of NT threads, in
+// the stage kernel's layout (hc_stage.hpp): phase 1 loads the state with a 2-cell halo and the
+// residual adjoint with a 1-cell halo into LDS (periodic wrap); phase 2 forms the LSQ solution
+// metric Q of the tile and its 1-cell ring in LDS (qmetric_cell's operations and order); phase 3
+// computes the contributions of every interior and periodic face touching the tile into LDS
+// (faces on a tile edge are computed by both tiles, with the same bits), then of the tile's
+// non-periodic boundary faces, one lane per face and differentiation direction; phase 4 gathers
+// each cell's direct adjoint and Qbar (AdjKernels::gather).  Neither Q nor the faces'
+// contributions go through HBM.
+template <int LAW, int BASIS, int FLUX, int TI, int TJ, int NT> struct AdjTile {
+    using K = AdjKernels<LAW, BASIS, FLUX>;
+    static constexpr int NV = LawTraits<LAW>::nv;
+    static constexpr int QH = TI + 4, QW = TJ + 4;   // state tile with 2-cell halo
+    static constexpr int MH = TI + 2, MW = TJ + 2;   // adjoint and metric tiles with 1-cell halo
+    static constexpr int NFI = (TI + 1) * TJ;   // i-face of node line li, row lj: li * TJ + lj
+    static constexpr int NFJ = TI * (TJ + 1);   // j-face of row li, node line lj: NFI + li * (TJ + 1) + lj
+    static constexpr int NF = NFI + NFJ;
+    struct Smem {
+        double q[NV][QH][QW];
+        double y[NV][MH][MW];
+        double Q[NV][2][MH][MW];
+        double c[6 * NV][NF];
+    };
+
+    // periodic wrap or clamp of a halo index (any distance: grids may be smaller than the halo)
+    HC_HD static int wrap(int g, int n, int per)
+    {
+        if (per) return ((g % n) + n) % n;
+        return g < 0 ? 0 : (g >= n ? n - 1 : g);
+    }
+
+    HC_HD static void load(const AdjArgs& a, Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        const long long N = a.N;
+        for (int k = tid; k < QH * QW; k += NT) {
+            const int li = k / QW, lj = k % QW;
+            const long long c = (long long)wrap(i0 - 2 + li, ni, a.m.per_i) * nj + wrap(j0 - 2 + lj, nj, a.m.per_j);
+            for (int v = 0; v < NV; ++v) s.q[v][li][lj] = a.q[v * N + c];
+            if (li >= 1 && li <= MH && lj >= 1 && lj <= MW)
+                for (int v = 0; v < NV; ++v) s.y[v][li - 1][lj - 1] = a.ybar[v * N + c];
+        }
+    }
+
+    // Q of the ring cell at metric position (mi, mj) = q-tile position (mi + 1, mj + 1): interior
+    // i-edges, interior j-edges, periodic i wrap, periodic j wrap, as qmetric_cell
+    HC_HD static void qmetric(const AdjArgs& a, Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        for (int k = tid; k < MH * MW; k += NT) {
+            const int mi = k / MW, mj = k % MW;
+            const int ri = i0 - 1 + mi, rj = j0 - 1 + mj;
+            if (!a.m.per_i && (ri < 0 || ri >= ni)) continue;
+            if (!a.m.per_j && (rj < 0 || rj >= nj)) continue;
+            const int gi = wrap(ri, ni, a.m.per_i), gj = wrap(rj, nj, a.m.per_j);
+            const int r = mi + 1, c = mj + 1;
+            double acc[NV][2];
+            for (int v = 0; v < NV; ++v) acc[v][0] = acc[v][1] = 0.0;
+            auto edge = [&](int rl, int cl, int rr, int cr, const double* base, long long nf, long long fidx) {
+                const double dx0 = base[FS_DX0 * nf + fidx];
+                const double dx1 = base[FS_DX1 * nf + fidx];
+                for (int v = 0; v < NV; ++v) {
+                    const double dq = s.q[v][rr][cr] - s.q[v][rl][cl];
+                    acc[v][0] = acc[v][0] + dq * dx0;
+                    acc[v][1] = acc[v][1] + dq * dx1;
+                }
+            };
+            if (gi > 0) edge(r - 1, c, r, c, a.m.fi, a.m.nfi, (long long)gi * nj + gj);
+            if (gi < ni - 1) edge(r, c, r + 1, c, a.m.fi, a.m.nfi, (long long)(gi + 1) * nj + gj);
+            if (gj > 0) edge(r, c - 1, r, c, a.m.fj, a.m.nfj, (long long)gi * (nj + 1) + gj);
+            if (gj < nj - 1) edge(r, c, r, c + 1, a.m.fj, a.m.nfj, (long long)gi * (nj + 1) + gj + 1);
+            if (a.m.per_i) {
+                if (gi == 0) edge(r - 1, c, r, c, a.m.fi, a.m.nfi, (long long)gj);
+                if (gi == ni - 1) edge(r, c, r + 1, c, a.m.fi, a.m.nfi, (long long)gj);
+            }
+            if (a.m.per_j) {
+                if (gj == 0) edge(r, c - 1, r, c, a.m.fj, a.m.nfj, (long long)gi * (nj + 1));
+                if (gj == nj - 1) edge(r, c, r, c + 1, a.m.fj, a.m.nfj, (long long)gi * (nj + 1));
+            }
+            for (int v = 0; v < NV; ++v) {
+                s.Q[v][0][mi][mj] = acc[v][0];
+                s.Q[v][1][mi][mj] = acc[v][1];
+            }
+        }
+    }
+
+    HC_HD static void faces(const AdjArgs& a, Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        for (int lf = tid; lf < NF; lf += NT) {
+            const bool isi = lf < NFI;
+            const int li = isi ? lf / TJ : (lf - NFI) / (TJ + 1);
+            const int lj = isi ? lf % TJ : (lf - NFI) % (TJ + 1);
+            const int gi = i0 + li, gj = j0 + lj;
+            long long f;
+            if (isi) {
+                if (gi > ni || gj >= nj) continue;
+                f = (long long)((gi == ni && a.m.per_i) ? 0 : gi) * nj + gj;
+            } else {
+                if (gi >= ni || gj > nj) continue;
+                f = a.m.nfi + (long long)gi * (nj + 1) + ((gj == nj && a.m.per_j) ? 0 : gj);
+            }
+            const FaceRef r = face_ref(a.m, f);
+            if (r.boundary) continue;
+            // metric positions of the left and right cells (q-tile positions one further in)
+            const int ml = isi ? li : li + 1, nl = isi ? lj + 1 : lj;
+            const int mr = li + 1, nr = lj + 1;
+            double yl[NV], yr[NV], ql[NV], qr[NV], Ql[NV][2], Qr[NV][2];
+            for (int v = 0; v < NV; ++v) {
+                yl[v] = s.y[v][ml][nl];
+                yr[v] = s.y[v][mr][nr];
+                ql[v] = s.q[v][ml + 1][nl + 1];
+                qr[v] = s.q[v][mr + 1][nr + 1];
+                for (int d = 0; d < 2; ++d) {
+                    Ql[v][d] = s.Q[v][d][ml][nl];
+                    Qr[v][d] = s.Q[v][d][mr][nr];
+                }
+            }
+            double mb;
+            K::face_contrib(a, r, yl, yr, ql, qr, Ql, Qr, &mb, [&](int k, double v) { s.c[k][lf] = v; });
+            // one tile owns each face's minf adjoint: its first node line inside the tile (the
+            // duplicate of a periodic wrap face at the far end is not the owner)
+            if (isi ? (li < TI && gi < ni) : (lj < TJ && gj < nj)) a.minf_face[f] = a.minf_face[f] + mb;
+        }
+        // the tile's non-periodic boundary faces (each belongs to this tile alone), one lane per
+        // face and direction: i-faces of node line 0 and ni, j-faces of node line 0 and nj
+        const int lbi0 = (i0 == 0 && !a.m.per_i) ? 0 : -1;
+        const int lbi1 = (!a.m.per_i && ni - i0 <= TI) ? ni - i0 : -1;
+        const int lbj0 = (j0 == 0 && !a.m.per_j) ? 0 : -1;
+        const int lbj1 = (!a.m.per_j && nj - j0 <= TJ) ? nj - j0 : -1;
+        if (lbi0 < 0 && lbi1 < 0 && lbj0 < 0 && lbj1 < 0) return;
+        for (int t = tid; t < 2 * (TI + TJ) * K::NDIR; t += NT) {
+            const int b = t / K::NDIR, k = t % K::NDIR;
+            int lf;
+            long long f;
+            if (b < 2 * TJ) {   // i-face of local node line lbi0 / lbi1, row lj
+                const int li = b < TJ ? lbi0 : lbi1, lj = b % TJ;
+                if (li < 0 || j0 + lj >= nj) continue;
+                lf = li * TJ + lj;
+                f = (long long)(i0 + li) * nj + j0 + lj;
+            } else {            // j-face of row li, local node line lbj0 / lbj1
+                const int bb = b - 2 * TJ;
+                const int lj = bb < TI ? lbj0 : lbj1, li = bb % TI;
+                if (lj < 0 || i0 + li >= ni) continue;
+                lf = NFI + li * (TJ + 1) + lj;
+                f = a.m.nfi + (long long)(i0 + li) * (nj + 1) + j0 + lj;
+            }
+            K::boundary_contrib(a, f, k, [&](int row, double v) { s.c[row][lf] = v; });
+        }
+    }
+
+    HC_HD static void cells(const AdjArgs& a, const Smem& s, int i0, int j0, int tid)
+    {
+        const int ni = a.m.ni, nj = a.m.nj;
+        for (int lc = tid; lc < TI * TJ; lc += NT) {
+            const int li = lc / TJ, lj = lc % TJ;
+            const int gi = i0 + li, gj = j0 + lj;
+            if (gi >= ni || gj >= nj) continue;
+            const int lfl[4] = {li * TJ + lj, (li + 1) * TJ + lj, NFI + li * (TJ + 1) + lj, NFI + li * (TJ + 1) + lj + 1};
+            K::gather(a, (long long)gi * nj + gj, [&](int row, int side) { return s.c[row][lfl[side]]; });
+        }
+    }
+};
+
+// tile shape of the fused adjoint kernel: 247 faces on 256 lanes, 70 KB of LDS in FP64 (two
+// work-groups per CU, the occupancy its registers allow)
+constexpr int ADJ_TI = 7, ADJ_TJ = 16, ADJ_NT = 256;
+
 #if defined(__HIPCC__)
+
+template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(ADJ_NT) k_adj_tile(AdjArgs a)
+{
+    using T = AdjTile<LAW, BASIS, FLUX, ADJ_TI, ADJ_TJ, ADJ_NT>;
+    __shared__ typename T::Smem s;
+    const int tiles_j = (a.m.nj + ADJ_TJ - 1) / ADJ_TJ;
+    const int i0 = (int)(blockIdx.x / tiles_j) * ADJ_TI, j0 = (int)(blockIdx.x % tiles_j) * ADJ_TJ;
+    T::load(a, s, i0, j0, threadIdx.x);
+    __syncthreads();
+    T::qmetric(a, s, i0, j0, threadIdx.x);
+    __syncthreads();
+    T::faces(a, s, i0, j0, threadIdx.x);
+    __syncthreads();
+    T::cells(a, s, i0, j0, threadIdx.x);
+}
 
 template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_tan_qmetric(AdjArgs a)
 {
@@ -597,29 +768,6 @@ template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k
 {
     const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
     if (c < a.N) AdjKernels<LAW, BASIS, FLUX>::tan_cell(a, c);
-}
-template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_qmetric(AdjArgs a)
-{
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (c < a.N) AdjKernels<LAW, BASIS, FLUX>::adj_qmetric(a, c);
-}
-template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_face(AdjArgs a)
-{
-    const long long f = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (f < a.nfaces) AdjKernels<LAW, BASIS, FLUX>::adj_face_interior(a, f);
-}
-// one thread per direction of each of the 2 nj + 2 ni boundary-face positions
-template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(64) k_adj_bface(AdjArgs a)
-{
-    using K = AdjKernels<LAW, BASIS, FLUX>;
-    const long long t = (long long)blockIdx.x * 64 + threadIdx.x;
-    const long long b = t / K::NDIR;
-    if (b < 2LL * (a.m.ni + a.m.nj)) K::adj_face_boundary(a, K::boundary_face(a.m, b), (int)(t % K::NDIR));
-}
-template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_cell(AdjArgs a)
-{
-    const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (c < a.N) AdjKernels<LAW, BASIS, FLUX>::adj_cell(a, c);
 }
 template <int LAW, int BASIS, int FLUX> __global__ void __launch_bounds__(256) k_adj_qedges(AdjArgs a)
 {

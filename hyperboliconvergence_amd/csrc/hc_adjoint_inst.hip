@@ -18,11 +18,8 @@ template <int L, int B, int F> hipError_t tangent(const hc::AdjArgs& a, hipStrea
 
 template <int L, int B, int F> hipError_t adjoint(const hc::AdjArgs& a, hipStream_t st)
 {
-    hipLaunchKernelGGL((hc::k_adj_qmetric<L, B, F>), dim3(blocks(a.N)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((hc::k_adj_face<L, B, F>), dim3(blocks(a.nfaces)), dim3(256), 0, st, a);
-    const long long nbt = 2LL * (a.m.ni + a.m.nj) * hc::AdjKernels<L, B, F>::NDIR;
-    hipLaunchKernelGGL((hc::k_adj_bface<L, B, F>), dim3((unsigned)((nbt + 63) / 64)), dim3(64), 0, st, a);
-    hipLaunchKernelGGL((hc::k_adj_cell<L, B, F>), dim3(blocks(a.N)), dim3(256), 0, st, a);
+    const long long tiles = (long long)((a.m.ni + hc::ADJ_TI - 1) / hc::ADJ_TI) * ((a.m.nj + hc::ADJ_TJ - 1) / hc::ADJ_TJ);
+    hipLaunchKernelGGL((hc::k_adj_tile<L, B, F>), dim3((unsigned)tiles), dim3(hc::ADJ_NT), 0, st, a);
     hipLaunchKernelGGL((hc::k_adj_qedges<L, B, F>), dim3(blocks(a.N)), dim3(256), 0, st, a);
     return hipGetLastError();
 }

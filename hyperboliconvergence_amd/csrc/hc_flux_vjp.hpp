@@ -16,15 +16,17 @@
 // differentiated: the adjoint trajectories run with a fixed time step.
 #pragma once
 
-#include "hc_point.hpp"
+#include "hc_dual.hpp"
 
 namespace hc {
 
 // reverse of euler_from_conserved / euler_from_primitive: adds to qb the adjoint of the
-// solution set given the adjoints of the state's u, v, r, p, h and a2
+// solution set given the adjoints of the state's u, v, r, p, h and a2 (inv_Rg = 1 / Rg; the
+// reverse sweeps multiply by reciprocals instead of dividing, since they need not reproduce any
+// forward rounding)
 template <int BASIS>
-HC_HD void euler_state_vjp(const Species<double>& sp, const double* q, const EState<double>& s, double ub, double vb,
-                           double rb, double pb, double hb, double a2b, double* qb)
+HC_HD void euler_state_vjp(const Species<double>& sp, double inv_Rg, const double* q, const EState<double>& s,
+                           double ub, double vb, double rb, double pb, double hb, double a2b, double* qb)
 {
     const double gam = sp.gamma, Rg = sp.Rg, gam1 = sp.gamma1;
     // h = a2 * gam1 + k, a2 = gam * Rg * t, k = 0.5 * v2
@@ -32,18 +34,18 @@ HC_HD void euler_state_vjp(const Species<double>& sp, const double* q, const ESt
     const double kb = hb;
     const double tb = a2b * (gam * Rg);
     const double v2b = 0.5 * kb;
+    const double inv_r = 1.0 / s.r;
+    // t = p / (r Rg): dt = dp / (r Rg) - t dr / r
+    pb = pb + tb * inv_r * inv_Rg;
+    rb = rb - tb * s.t * inv_r;
     if constexpr (BASIS == BASIS_PRIMITIVE) {
-        // t = p / (r Rg); v2 = u^2 + v^2
-        pb = pb + tb / (s.r * Rg);
-        rb = rb - tb * s.t / s.r;
+        // v2 = u^2 + v^2
         qb[0] += ub + 2.0 * s.u * v2b;
         qb[1] += vb + 2.0 * s.v * v2b;
         qb[2] += rb;
         qb[3] += pb;
     } else {
-        // t = p / (Rg r); p = (gam - 1)(re - r k); u = ru / r, v = rv / r
-        pb = pb + tb / (Rg * s.r);
-        rb = rb - tb * s.t / s.r;
+        // p = (gam - 1)(re - r k); u = ru / r, v = rv / r
         const double k = 0.5 * s.v2;
         const double reb = pb * (gam - 1.0);
         rb = rb - pb * (gam - 1.0) * k;
@@ -51,11 +53,10 @@ HC_HD void euler_state_vjp(const Species<double>& sp, const double* q, const ESt
         const double v2b2 = v2b + 0.5 * kb2;
         ub = ub + 2.0 * s.u * v2b2;
         vb = vb + 2.0 * s.v * v2b2;
-        const double r1 = 1.0 / q[2];
-        qb[0] += ub * r1;
-        qb[1] += vb * r1;
+        qb[0] += ub * inv_r;
+        qb[1] += vb * inv_r;
         const double r1b = ub * q[0] + vb * q[1];
-        qb[2] += rb - r1b * r1 * r1;
+        qb[2] += rb - r1b * inv_r * inv_r;
         qb[3] += reb;
     }
 }
@@ -71,9 +72,9 @@ HC_HD double ms_m4_d(int sign, double M, double beta)
 HC_HD double ms_p1_d(int sign, double M)
 {
     const double eps = NumLimits<double>::max();
-    const double b = hc_fabs(M) + eps;
-    const double r = (ms_m1(sign, M) + eps) / b;
-    return (ms_m1_d(sign, M) - r * (M < 0.0 ? -1.0 : 1.0)) / b;
+    const double inv_b = 1.0 / (hc_fabs(M) + eps);
+    const double r = (ms_m1(sign, M) + eps) * inv_b;
+    return (ms_m1_d(sign, M) - r * (M < 0.0 ? -1.0 : 1.0)) * inv_b;
 }
 HC_HD double ms_p3_d(int sign, double M) { return ms_m2_d(sign, M) * (double(sign) * 2.0 - M) - ms_m2(sign, M); }
 // d P5 / d M and d P5 / d alpha
@@ -84,6 +85,30 @@ HC_HD void ms_p5_d(int sign, double M, double alpha, double& dM, double& dalpha)
     const double da = ms_m2_d(sign, M), db = ms_m2_d(-sign, M);
     dM = ms_p3_d(sign, M) - s16 * alpha * (a * b + M * da * b + M * a * db);
     dalpha = -s16 * M * a * b;
+}
+
+// Limiter value and its partial derivatives in both arguments: the smooth van Albada limiter
+// (limiters/limiter.ipp:161-172 without the sign switch) by hand with one reciprocal, the
+// others from Dual<2> numbers.  The value has the forward path's bits.
+HC_HD double limit_d(int lim, double a, double b, double& da, double& db)
+{
+    if (lim == LIM_SMOOTHVANALBADA) {
+        const double eps = NumLimits<double>::min();
+        const double num = b * (a * a) + a * (b * b);
+        const double den = a * a + b * b + eps;
+        const double f = num / den;
+        const double inv = 1.0 / den;
+        da = (2.0 * a * b + b * b - f * (2.0 * a)) * inv;
+        db = (a * a + 2.0 * a * b - f * (2.0 * b)) * inv;
+        return f;
+    }
+    Dual<2> x(a), y(b);
+    x.d[0] = 1.0;
+    y.d[1] = 1.0;
+    const Dual<2> r = limit(lim, x, y);
+    da = r.d[0];
+    db = r.d[1];
+    return r.v;
 }
 
 // AUSM+up face flux (euler_ausmpup on the states of ql, qr): adds (dF/dql)^T w to qlb,
@@ -150,6 +175,7 @@ HC_HD double ausmpup_vjp(const Species<double>& sp, int vscale, int pscale, cons
     const double rup = m1l * sl.r + m1r * sr.r;
     const double mdot = ua * rup;
     // ---- reverse ----
+    const double inv_as = 1.0 / as;
     // F0 = mdot u_up + pa n0, F1 = mdot v_up + pa n1, F2 = mdot, F3 = mdot h_up
     const double mdotb = w[0] * (m1l * sl.u + m1r * sr.u) + w[1] * (m1l * sl.v + m1r * sr.v) + w[2] +
                          w[3] * (m1l * sl.h + m1r * sr.h);
@@ -195,9 +221,10 @@ HC_HD double ausmpup_vjp(const Species<double>& sp, int vscale, int pscale, cons
     mrb += Sb * (mrsub * ms_m4_d(-1, mr, beta) + mrsup * ms_m1_d(-1, mr));
     const double mpb = -Sb;
     // mp = Kp mpc delp / D, D = fa_p ra as as
-    const double mpcb = mpb * Kp * delp / D;
-    const double delpb = mpb * Kp * mpc / D;
-    const double Db = -mpb * mp / D;
+    const double inv_D = 1.0 / D;
+    const double mpcb = mpb * Kp * delp * inv_D;
+    const double delpb = mpb * Kp * mpc * inv_D;
+    const double Db = -mpb * mp * inv_D;
     double fa_pb = Db * ra * as * as;
     rab += Db * fa_p * as * as;
     asb += Db * 2.0 * fa_p * ra * as;
@@ -218,28 +245,30 @@ HC_HD double ausmpup_vjp(const Species<double>& sp, int vscale, int pscale, cons
     if (ba == 1) minfb += m0_ab;
     else if (ba == 0) {
         if (mx_m0) m0b += m0_ab;
-        else asb += m0_ab * (-mu / as);   // mu = lref / (as dt)
+        else asb += m0_ab * (-mu * inv_as);   // mu = lref / (as dt)
     }
     // m0 = sqrt(0.5 (ml^2 + mr^2))
     const double Y = m0b * (0.5 / m0);
     mlb += Y * ml;
     mrb += Y * mr;
     // ml = unl / as, mr = unr / as
-    unlb += mlb / as;
-    unrb += mrb / as;
-    asb -= (mlb * ml + mrb * mr) / as;
+    unlb += mlb * inv_as;
+    unrb += mrb * inv_as;
+    asb -= (mlb * ml + mrb * mr) * inv_as;
     // as = fmin(ars, als)
     const double arsb = as_r ? asb : 0.0, alsb = as_r ? 0.0 : asb;
     // ra = (rl + rr) / 2
     rlb += 0.5 * rab;
     rrb += 0.5 * rab;
     // als = als2 / fmax(unl, sqrt(als2)), ars = ars2 / fmax(-unr, sqrt(ars2))
-    double als2b = alsb / dal;
-    const double dalb = -alsb * als / dal;
+    const double inv_dal = 1.0 / dal;
+    double als2b = alsb * inv_dal;
+    const double dalb = -alsb * als * inv_dal;
     if (dal_u) unlb += dalb;
     else als2b += dalb * (0.5 / sqal);
-    double ars2b = arsb / dar;
-    const double darb = -arsb * ars / dar;
+    const double inv_dar = 1.0 / dar;
+    double ars2b = arsb * inv_dar;
+    const double darb = -arsb * ars * inv_dar;
     if (dar_u) unrb -= darb;
     else ars2b += darb * (0.5 / sqar);
     hlb += als2b * ascoeff;
@@ -249,8 +278,9 @@ HC_HD double ausmpup_vjp(const Species<double>& sp, int vscale, int pscale, cons
     vlb += unlb * f.n1;
     urb += unrb * f.n0;
     vrb += unrb * f.n1;
-    euler_state_vjp<BASIS>(sp, ql, sl, ulb, vlb, rlb, plb, hlb, 0.0, qlb);
-    euler_state_vjp<BASIS>(sp, qr, sr, urb, vrb, rrb, prb, hrb, 0.0, qrb);
+    const double inv_Rg = 1.0 / sp.Rg;
+    euler_state_vjp<BASIS>(sp, inv_Rg, ql, sl, ulb, vlb, rlb, plb, hlb, 0.0, qlb);
+    euler_state_vjp<BASIS>(sp, inv_Rg, qr, sr, urb, vrb, rrb, prb, hrb, 0.0, qrb);
     return minfb;
 }
 

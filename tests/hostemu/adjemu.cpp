@@ -14,7 +14,7 @@ namespace {
 
 struct AdjEmu {
     GeomOutput g;
-    std::vector<double> q, qd, bref, brefd, Q, Qd, F, Fd, res, resd, ybar, fout, minf_face, brefbar, qbar, Qbar;
+    std::vector<double> q, qd, bref, brefd, Q, Qd, F, Fd, res, resd, ybar, minf_face, brefbar, qbar, Qbar;
     AdjArgs a{};
 };
 
@@ -66,12 +66,21 @@ template <int LAW, int BASIS, int FLUX> void run_tangent(AdjArgs& a)
     for (long long c = 0; c < a.N; ++c) K::tan_cell(a, c);
 }
 
+// the GPU's launches in order: the tiles (each phase for every thread of the work-group before the
+// next), the metric transpose
 template <int LAW, int BASIS, int FLUX> void run_adjoint(AdjArgs& a)
 {
     using K = AdjKernels<LAW, BASIS, FLUX>;
-    for (long long c = 0; c < a.N; ++c) K::adj_qmetric(a, c);
-    for (long long f = 0; f < a.nfaces; ++f) K::adj_face(a, f);
-    for (long long c = 0; c < a.N; ++c) K::adj_cell(a, c);
+    using T = AdjTile<LAW, BASIS, FLUX, ADJ_TI, ADJ_TJ, ADJ_NT>;
+    std::vector<typename T::Smem> smem(1);
+    const int tiles_i = (a.m.ni + ADJ_TI - 1) / ADJ_TI, tiles_j = (a.m.nj + ADJ_TJ - 1) / ADJ_TJ;
+    for (int t = 0; t < tiles_i * tiles_j; ++t) {
+        const int i0 = (t / tiles_j) * ADJ_TI, j0 = (t % tiles_j) * ADJ_TJ;
+        for (int tid = 0; tid < ADJ_NT; ++tid) T::load(a, smem[0], i0, j0, tid);
+        for (int tid = 0; tid < ADJ_NT; ++tid) T::qmetric(a, smem[0], i0, j0, tid);
+        for (int tid = 0; tid < ADJ_NT; ++tid) T::faces(a, smem[0], i0, j0, tid);
+        for (int tid = 0; tid < ADJ_NT; ++tid) T::cells(a, smem[0], i0, j0, tid);
+    }
     for (long long c = 0; c < a.N; ++c) K::adj_qedges(a, c);
 }
 
@@ -133,13 +142,11 @@ int emu_adjoint_residual(const hc_case_desc* cs, int ni, int nj, const double* n
     e.ybar.resize((size_t)nv * N);
     for (long long c = 0; c < N; ++c)
         for (int v = 0; v < nv; ++v) e.ybar[(size_t)v * N + c] = ybar[c * nv + v];
-    e.fout.assign((size_t)6 * nv * e.a.nfaces, 0.0);
     e.minf_face.assign((size_t)e.a.nfaces, 0.0);
     e.brefbar.assign(e.bref.size(), 0.0);
     e.qbar.assign((size_t)nv * N, 0.0);
     e.Qbar.assign((size_t)2 * nv * N, 0.0);
     e.a.ybar = e.ybar.data();
-    e.a.fout = e.fout.data();
     e.a.minf_face = e.minf_face.data();
     e.a.brefbar = e.brefbar.data();
     e.a.qbar = e.qbar.data();
