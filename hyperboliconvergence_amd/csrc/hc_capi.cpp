@@ -134,6 +134,15 @@ __global__ void __launch_bounds__(256) k_triad(double2* __restrict__ a, const do
     }
 }
 
+// a = s b on 8-byte elements, one per lane: the stage kernels' access width, run once per triad
+// call so that a counter pass (FETCH_SIZE / WRITE_SIZE) sees a kernel of known traffic at both widths
+__global__ void __launch_bounds__(256) k_scale8(double* __restrict__ a, const double* __restrict__ b, double s,
+                                                long long n)
+{
+    const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = s * b[i];
+}
+
 __global__ void __launch_bounds__(256) k_fill(double* a, double v, long long n)
 {
     for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256) a[i] = v;
@@ -1372,6 +1381,10 @@ int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps)
     float ms = 0.f;
     HC_HIP(hipEventElapsedTime(&ms, w.e0, w.e1));
     *gbps = 48.0 * (double)n2 * reps / (ms * 1e-3) / 1e9;
+    hipLaunchKernelGGL(k_scale8, dim3((unsigned)((2 * n2 + 255) / 256)), dim3(256), 0, ctx->stream, w.a, w.b, 3.0,
+                       2 * n2);
+    HC_HIP(hipGetLastError());
+    HC_HIP(hipStreamSynchronize(ctx->stream));
     return HC_OK;
 }
 

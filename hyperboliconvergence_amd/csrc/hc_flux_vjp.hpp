@@ -88,7 +88,7 @@ HC_HD void ms_p5_d(int sign, double M, double alpha, double& dM, double& dalpha)
 }
 
 // Limiter value and its partial derivatives in both arguments: the smooth van Albada limiter
-// (limiters/limiter.ipp:161-172 without the sign switch) by hand with one reciprocal, the
+// (limiters/limiter.ipp:27-38 without the sign switch) by hand with one reciprocal, the
 // others from Dual<2> numbers.  The value has the forward path's bits.
 HC_HD double limit_d(int lim, double a, double b, double& da, double& db)
 {

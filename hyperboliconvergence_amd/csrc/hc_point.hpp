@@ -806,29 +806,30 @@ HC_HD Flux4<double> face_flux_mixed(const Species<double>& sp, const Species<flo
 
 // ---------------- limiters (limiters/limiter.ipp) ----------------
 
-template <class R> HC_HD R limit(int lim, R a, R b)
+// limiter L of the pair (a, b): a the centred slope, b the upwind-biased one
+template <int L, class R> HC_HD R limit_fn(R a, R b)
 {
-    switch (lim) {
-    case LIM_NOLIMIT1: return R(0);                                   // :143-147
-    case LIM_NOLIMIT2: return R(0.5) * (a + b);                       // :149-153
-    case LIM_NOLIMIT3: return R(0.33333333333333333) * (R(2.) * a + b);   // :155-159
-    case LIM_VANALBADA2: {                                            // :161-172
+    if constexpr (L == LIM_NOLIMIT1) {   // limiter.ipp:9-13
+        return R(0);
+    } else if constexpr (L == LIM_NOLIMIT2) {   // limiter.ipp:15-19
+        return R(0.5) * (a + b);
+    } else if constexpr (L == LIM_NOLIMIT3) {   // limiter.ipp:21-25
+        return R(0.33333333333333333) * (R(2.) * a + b);
+    } else if constexpr (L == LIM_VANALBADA2) {   // limiter.ipp:27-38
         const R eps = NumLimits<R>::min();
         return a * b < R(0) ? R(0) : (b * (a * a) + a * (b * b)) / (a * a + b * b + eps);
-    }
-    case LIM_MINMOD2:                                                  // :174-182
+    } else if constexpr (L == LIM_MINMOD2) {   // limiter.ipp:40-48
         return a > R(0) ? hc_fmin(a, hc_fmax(b, R(0.))) : hc_fmax(a, hc_fmin(b, R(0.)));
-    case LIM_SUPERBEE2:                                                // :184-197
+    } else if constexpr (L == LIM_SUPERBEE2) {   // limiter.ipp:50-63
         return a > R(0) ? hc_fmax(R(0.), hc_fmax(hc_fmin(R(2) * a, b), hc_fmin(R(2) * b, a)))
                         : hc_fmin(R(0.), hc_fmin(hc_fmax(R(2) * a, b), hc_fmax(R(2) * b, a)));
-    case LIM_MC2:                                                      // :199-214
+    } else if constexpr (L == LIM_MC2) {   // limiter.ipp:65-80
         return a > R(0) ? hc_fmax(R(0.), hc_fmin(R(0.5) * (a + b), hc_fmin(R(2) * b, R(2) * a)))
                         : hc_fmin(R(0.), hc_fmax(R(0.5) * (a + b), hc_fmax(R(2) * b, R(2) * a)));
-    case LIM_VANLEER2: {                                               // :216-224
+    } else if constexpr (L == LIM_VANLEER2) {   // limiter.ipp:82-90
         const R eps = NumLimits<R>::min();
         return (b + hc_fabs(b)) / (a + hc_fabs(b) + eps);
-    }
-    case LIM_CADA3: {                                                  // :226-296
+    } else if constexpr (L == LIM_CADA3) {   // limiter.ipp:92-161
         const R t = R(0.0);
         const R gam = R(1.5);
         const R c1 = R(-2. / 7.);
@@ -851,11 +852,47 @@ template <class R> HC_HD R limit(int lim, R a, R b)
         }
         if (a <= R(0)) w = -w;
         return w;
-    }
-    default: {   // smooth van Albada: VanAlbada2 without the a*b<0 switch
+    } else {   // smooth van Albada: VanAlbada2 without the a*b<0 switch
         const R eps = NumLimits<R>::min();
         return (b * (a * a) + a * (b * b)) / (a * a + b * b + eps);
     }
+}
+
+#define HC_FOR_EACH_LIMITER(X)                                                                            \
+    X(LIM_NOLIMIT1) X(LIM_NOLIMIT2) X(LIM_NOLIMIT3) X(LIM_VANALBADA2) X(LIM_MINMOD2) X(LIM_SUPERBEE2) X(LIM_MC2) \
+    X(LIM_VANLEER2) X(LIM_CADA3)
+
+// limiters/limiter.ipp:10-161 by run-time id (an unknown id is the smooth van Albada limiter)
+template <class R> HC_HD R limit(int lim, R a, R b)
+{
+    switch (lim) {
+#define HC_LIM_CASE(L) \
+    case L: return limit_fn<L>(a, b);
+        HC_FOR_EACH_LIMITER(HC_LIM_CASE)
+#undef HC_LIM_CASE
+    default: return limit_fn<LIM_SMOOTHVANALBADA>(a, b);
+    }
+}
+
+// the limiter of every pair (a[v], bl[v]) and (a[v], br[v]), v < N: one branch on the id for the
+// face, then straight-line code over the pairs (the same operations as limit, pair by pair)
+template <class R, int N> HC_HD void limit_pairs(int lim, const R* a, const R* bl, const R* br, R* sl, R* sr)
+{
+    switch (lim) {
+#define HC_LIM_CASE(L)                           \
+    case L:                                      \
+        for (int v = 0; v < N; ++v) {            \
+            sl[v] = limit_fn<L>(a[v], bl[v]);    \
+            sr[v] = limit_fn<L>(a[v], br[v]);    \
+        }                                        \
+        return;
+        HC_FOR_EACH_LIMITER(HC_LIM_CASE)
+#undef HC_LIM_CASE
+    default:
+        for (int v = 0; v < N; ++v) {
+            sl[v] = limit_fn<LIM_SMOOTHVANALBADA>(a[v], bl[v]);
+            sr[v] = limit_fn<LIM_SMOOTHVANALBADA>(a[v], br[v]);
+        }
     }
 }
 

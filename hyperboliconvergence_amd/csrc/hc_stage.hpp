@@ -396,35 +396,40 @@ struct StageTile {
             if (a.limiter == LIM_NOLIMIT1)
                 return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, ql, qr);
             const RF n0 = RF(f.n0), n1 = RF(f.n1);
-            double qlf[NV], qrf[NV];
+            RF dqdx[NV], gl[NV], gr[NV], sl[NV], sr[NV];
             for (int v = 0; v < NV; ++v) {
                 const RF dq = RF(qr[v] - ql[v]);
-                const RF dqdx = dq * g.s1;
+                dqdx[v] = dq * g.s1;
                 const RF c0 = dq * g.dx0;
                 const RF c1 = dq * g.dx1;
-                const RF gl = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
-                const RF gr = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
-                const RF slope_l = limit(a.limiter, dqdx, gl);
-                const RF slope_r = limit(a.limiter, dqdx, gr);
-                qlf[v] = ql[v] + double(slope_l * g.dl);
-                qrf[v] = qr[v] - double(slope_r * g.dr);
+                gl[v] = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, n0, n1);
+                gr[v] = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, n0, n1);
+            }
+            limit_pairs<RF, NV>(a.limiter, dqdx, gl, gr, sl, sr);
+            double qlf[NV], qrf[NV];
+            for (int v = 0; v < NV; ++v) {
+                qlf[v] = ql[v] + double(sl[v] * g.dl);
+                qrf[v] = qr[v] - double(sr[v] * g.dr);
             }
             return face_flux_mixed<LAW, BASIS, FLUX>(a.sp, a.spf, a.vscale, a.pscale, f, qlf, qrf);
         } else {
             const FaceG<RF> f{g.area, g.n0, g.n1};
             if (a.limiter == LIM_NOLIMIT1) return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, ql, qr);
-            RF qlf[NV], qrf[NV];
+            RF dqdx[NV], gl[NV], gr[NV], sl[NV], sr[NV];
             for (int v = 0; v < NV; ++v) {
                 const RF dq = qr[v] - ql[v];
-                const RF dqdx = dq * g.s1;
+                dqdx[v] = dq * g.s1;
                 const RF c0 = dq * g.dx0;
                 const RF c1 = dq * g.dx1;
-                const RF gl = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
-                const RF gr = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
-                const RF slope_l = limit(a.limiter, dqdx, gl);
-                const RF slope_r = limit(a.limiter, dqdx, gr);
-                qlf[v] = ql[v] + slope_l * g.dl;
-                qrf[v] = qr[v] - slope_r * g.dr;
+                gl[v] = directional(g.fl, Ql[v][0] - c0, Ql[v][1] - c1, f.n0, f.n1);
+                gr[v] = directional(g.fr, Qr[v][0] - c0, Qr[v][1] - c1, f.n0, f.n1);
+            }
+            // one branch on the limiter per face, straight-line code over the variables
+            limit_pairs<RF, NV>(a.limiter, dqdx, gl, gr, sl, sr);
+            RF qlf[NV], qrf[NV];
+            for (int v = 0; v < NV; ++v) {
+                qlf[v] = ql[v] + sl[v] * g.dl;
+                qrf[v] = qr[v] - sr[v] * g.dr;
             }
             return face_flux<RF, LAW, BASIS, FLUX>(a.spf, a.vscale, a.pscale, f, qlf, qrf);
         }

@@ -188,8 +188,10 @@ int hc_device_state(hc_context* ctx, double** dptr);
 /*
  * STREAM triad a[i] = b[i] + s*c[i] over three device arrays of n doubles (n >= 2), repeated
  * reps times with 16-byte loads and stores; returns the attainable HBM bandwidth in GB/s
- * (24 bytes per double of the even part of n per repetition).  Used as the attainable-bandwidth
- * denominator and to calibrate memory counters.  No counterpart in the reference.
+ * (24 bytes per double of the even part of n per repetition).  Then, once, a[i] = s*b[i] over the
+ * same even part with 8-byte accesses (the stage kernels' width).  Used as the attainable-bandwidth
+ * denominator and to calibrate memory counters at both access widths.  No counterpart in the
+ * reference.
  */
 int hc_stream_triad(hc_context* ctx, long long n, int reps, double* gbps);
 
