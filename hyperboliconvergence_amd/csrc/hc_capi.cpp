@@ -394,12 +394,11 @@ int build_tiles(hc_context* ctx)
         std::vector<TileRef> tab;
         tab.reserve((size_t)(2 * nt));
         if (ctx->interleave) {
-            for (int g = 0; g < nc; ++g) {
-                if (ctx->cases[g].geo != g) continue;   // groups in the order of their first case
-                for (int lt = 0; lt < ctx->cases[g].ntiles[k]; ++lt)
-                    for (int ci = g; ci < nc; ++ci)
-                        if (ctx->cases[ci].geo == g) tab.push_back(TileRef{ci, lt});
-            }
+            std::vector<std::vector<int>> members(nc);   // cases of each group, keyed by its first case
+            for (int ci = 0; ci < nc; ++ci) members[ctx->cases[ci].geo].push_back(ci);
+            for (int g = 0; g < nc; ++g)
+                for (int lt = 0; lt < ctx->cases[g].ntiles[k] && !members[g].empty(); ++lt)
+                    for (int ci : members[g]) tab.push_back(TileRef{ci, lt});
         } else {
             for (int ci = 0; ci < nc; ++ci)
                 for (int lt = 0; lt < ctx->cases[ci].ntiles[k]; ++lt) tab.push_back(TileRef{ci, lt});
