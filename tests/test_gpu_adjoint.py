@@ -116,8 +116,22 @@ def _adjoint_grad(cfg, K, C):
     return qK, g, q0b, bb, mb
 
 
-def test_trajectory_adjoint_is_transpose_of_trajectory_tangent():
+def _set_rk(cfg, rk):
+    n, a, b = po.rk_table(rk)
+    cs = cfg["case"]
+    cs.nstages = n
+    for r in range(6):
+        cs.beta[r] = b[r]
+        for k in range(6):
+            cs.alpha[r][k] = a[r, k]
+
+
+@pytest.mark.parametrize("rk", [33, 11, 22, 34, 44])
+def test_trajectory_adjoint_is_transpose_of_trajectory_tangent(rk):
+    """Every Runge-Kutta table (SSP 1-4 stages and classical RK4, whose rows hold zeros): the
+    backward sweep's fused stage adjoints against the trajectory tangent."""
     cfg = _cfg()
+    _set_rk(cfg, rk)
     K = cfg["nsteps"]
     qK, g, q0b, bb, mb = _adjoint_grad(cfg, K, 4)
     rng = np.random.default_rng(2)
