@@ -417,21 +417,36 @@ def main_adjoint(a):
     dt = 0.8 / float(np.max(lam / cases.cell_volumes(nodes)))
     cfg["case"].cfl = dt
     ctx.set_case(cfg["case"])
-    ctx.set_tiles(0)
+    # the stage kernels' tile configuration: the fastest over a short forward run of each (the
+    # trajectory entry points take the context's configuration as it is, they do not tune)
+    tms = []
+    for k in range(hca.NTILECFG):
+        ctx.set_tiles(k)
+        ctx.set_boundary_refs(bref)
+        ctx.set_state(q0)
+        ctx.adjoint_forward(C, C)
+        ctx.synchronize()
+        ctx.set_state(q0)
+        t0 = time.perf_counter()
+        ctx.adjoint_forward(2 * C, C)
+        ctx.synchronize()
+        tms.append(time.perf_counter() - t0)
+    tiles = int(np.argmin(tms))
+    ctx.set_tiles(tiles)
 
     def run(qs, minf=None):
         if minf is not None:
             cs = type(cfg["case"]).from_buffer_copy(cfg["case"])
             cs.sp.minf = minf
             ctx.set_case(cs)
-            ctx.set_tiles(0)
+            ctx.set_tiles(tiles)
         ctx.set_boundary_refs(bref)
         ctx.set_state(qs)
         ctx.adjoint_forward(K, C)
         out = ctx.get_state()
         if minf is not None:
             ctx.set_case(cfg["case"])
-            ctx.set_tiles(0)
+            ctx.set_tiles(tiles)
         return out
 
     run(q0)   # warm-up
@@ -468,13 +483,13 @@ def main_adjoint(a):
         cs = type(cfg["case"]).from_buffer_copy(cfg["case"])
         cs.sp.minf = m
         ctx.set_case(cs)
-        ctx.set_tiles(0)
+        ctx.set_tiles(tiles)
         ctx.set_boundary_refs(np.broadcast_to(f, (nb, 4)).copy())
         ctx.set_state(f[None, None, :] + pert)
         ctx.adjoint_forward(K, C)
         out = cases.wall_force_x(nodes, ctx.get_state())
         ctx.set_case(cfg["case"])
-        ctx.set_tiles(0)
+        ctx.set_tiles(tiles)
         return out
 
     hM = 1e-7
@@ -504,7 +519,8 @@ def main_adjoint(a):
                     "rel_err_vs_fd_h1e-6": rel(ad, fd[1e-6]), "rel_err_vs_fd_h1e-5": rel(ad, fd[1e-5]),
                     "rel_err_vs_tangent": rel(ad, tan), "dJdminf": mb, "dJdminf_fd_h1e-6": fdm,
                     "dJdminf_rel_err": rel(mb, fdm), "dJdM_freestream": ad_M, "dJdM_freestream_fd_h1e-6": fd_M,
-                    "dJdM_freestream_rel_err": rel(ad_M, fd_M), "bref_bar_norm": float(np.linalg.norm(bb))},
+                    "dJdM_freestream_rel_err": rel(ad_M, fd_M), "bref_bar_norm": float(np.linalg.norm(bb)),
+                    "tile_shape": tiles, "tile_shape_s_per_2C_steps": [float(x) for x in tms]},
         "status": int(st),
     }
     print(json.dumps(out), flush=True)

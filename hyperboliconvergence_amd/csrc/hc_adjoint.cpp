@@ -24,7 +24,7 @@ struct AdjState {
     int nv = 0, ns = 0;
     // residual-level work arrays
     double *Q = nullptr, *Qd = nullptr, *F = nullptr, *Fd = nullptr, *res = nullptr, *resd = nullptr;
-    double *minf_face = nullptr, *brefbar = nullptr, *qbar = nullptr, *Qbar = nullptr;
+    double *minf_face = nullptr, *qbar = nullptr, *Qbar = nullptr;
     double *ybar_in = nullptr, *qd_in = nullptr, *brefd_in = nullptr, *part = nullptr, *scal = nullptr;
     // trajectory arrays
     std::vector<double*> ckpt;       // checkpoints, one state each
@@ -145,7 +145,7 @@ int ensure(hc_context* ctx)
     if ((r = alloc(ctx, s.Q, 2 * nv * N)) || (r = alloc(ctx, s.Qd, 2 * nv * N)) || (r = alloc(ctx, s.F, (nv + 1) * nF)) ||
         (r = alloc(ctx, s.Fd, nv * nF)) || (r = alloc(ctx, s.res, (nv + 1) * N)) || (r = alloc(ctx, s.resd, nv * N)) ||
         (r = alloc(ctx, s.minf_face, nF)) ||
-        (r = alloc(ctx, s.brefbar, (long long)c.brows * nv)) || (r = alloc(ctx, s.qbar, nv * N)) ||
+        (r = alloc(ctx, s.qbar, nv * N)) ||
         (r = alloc(ctx, s.Qbar, 2 * nv * N)) || (r = alloc(ctx, s.ybar_in, nv * N)) || (r = alloc(ctx, s.qd_in, nv * N)) ||
         (r = alloc(ctx, s.brefd_in, (long long)c.brows * nv)) || (r = alloc(ctx, s.part, SUM_BLOCKS)) ||
         (r = alloc(ctx, s.scal, 4)) || (r = alloc(ctx, s.qbn, nv * N)) ||
@@ -188,7 +188,7 @@ AdjArgs base_adj(hc_context* ctx)
     a.res = s.res;
     a.resd = s.resd;
     a.minf_face = s.minf_face;
-    a.brefbar = s.brefbar;
+    a.brefbar = s.brefbar_tot;
     a.qbar = s.qbar;
     a.Qbar = s.Qbar;
     return a;
@@ -310,7 +310,7 @@ void release_adjoint(hc_context* ctx)
 {
     if (!ctx->adj) return;
     AdjState& s = *ctx->adj;
-    for (double** p : {&s.Q, &s.Qd, &s.F, &s.Fd, &s.res, &s.resd, &s.minf_face, &s.brefbar, &s.qbar, &s.Qbar,
+    for (double** p : {&s.Q, &s.Qd, &s.F, &s.Fd, &s.res, &s.resd, &s.minf_face, &s.qbar, &s.Qbar,
                        &s.ybar_in, &s.qd_in, &s.brefd_in, &s.part, &s.scal, &s.qbn, &s.brefbar_tot, &s.qtmp})
         free_d(*p);
     for (auto* v : {&s.ckpt, &s.seg, &s.stage, &s.a, &s.yb, &s.rd, &s.qdot})
