@@ -577,11 +577,15 @@ struct StageTile {
 
     // ---- phase 4: owner-computes residual accumulation and the stage update ----
     // red: the cell's lambda/vol for the global-dt reduction of stage 0 (0 otherwise)
-    // Cell prefetch (FP64 kernels with the geometry prefetch, which have the registers to spare at
-    // 4 waves per SIMD): the cell phase's inputs of the lane's cell (step state, the accumulator
-    // read first, volume, stage-0 spectral radius) are loaded into registers when the face phase
-    // opens, so the cell phase does not start with a round trip to memory
-    static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double);
+    // Cell prefetch: the cell phase's inputs of the lane's cell (step state, the accumulator read
+    // first, volume, stage-0 spectral radius) are loaded into registers when the face phase opens,
+    // so the cell phase does not start with a round trip to memory.  The 20 registers it holds
+    // through the face phase decide the occupancy of the FP64 7x16 kernels: without it the
+    // central, Rusanov, Roe-family and SLAU kernels fit 96 VGPRs (five waves per SIMD, which the
+    // 31 KB of LDS per work-group also allows; the 96-case batch runs 4.4 % faster, the 512x256
+    // cylinder 3.5 %), while AUSM+up needs more than 96 either way (four waves), so only its
+    // kernels keep the prefetch (the 4096^2 vortex runs 3.7 % slower without it).
+    static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double) && LAW == LAW_EULER && FLUX == FLUX_AUSMPUP;
     struct CellIn {
         double q0[NV], acc[NV], vol, lam0;
     };
