@@ -22,6 +22,12 @@ Other workloads (--workload): c1 scalar advection 128^2, c2 the 512 x 256 cylind
 flow direction, euler/fluxes/slau.ipp:18, 123-140, and that case leaves the finite numbers within
 150 steps), c5 the adjoint case.
 
+roofline.traffic, the HBM bytes of one stage-kernel launch, is measured in the same invocation:
+this process runs the measurement in a child process and then two short child runs under
+`rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE) on the tile configuration and precision the
+measurement reported, each counter calibrated on the 8-byte scale kernel of the bandwidth probe
+(--traffic file uses the committed profiles/traffic.json figure instead).
+
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c4|c1|c2|c3|c5]
 """
 import argparse
@@ -34,6 +40,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+TRIAD_DOUBLES = {"c4": 1 << 27}   # bandwidth probe size per workload (1 << 26 otherwise)
 
 
 def rel_err_conserved(q, qref, law, basis):
@@ -98,13 +105,14 @@ def first_nonfinite_step(cfg, steps, chunk=50):
     return None
 
 
-def run_gpu(cfg, precision, steps, warmup, graph_steps, uniform=1):
+def run_gpu(cfg, precision, steps, warmup, graph_steps, uniform=1, tiles=-1):
     import hyperboliconvergence_amd as hca
     ctx = hca.Context(0)
     ctx.set_mesh(cfg["nodes"])
     ctx.set_case(cfg["case"])
     ctx.set_precision(precision)
     ctx.set_uniform_geometry(uniform)
+    ctx.set_tiles(tiles)
     ctx.set_boundary_refs(cfg["bref"])
     ctx.set_state(cfg["q"])
     ctx.set_graph_steps(graph_steps)
@@ -178,6 +186,87 @@ def traffic_from_profiles(workload, precision, tile_cfg, uniform=False):
         return None
 
 
+def traffic_live(a, precision, tile_cfg, triad_n, timeout_s=240):
+    """HBM bytes per stage-kernel launch measured in this run: two child runs of this script under
+    `rocprofv3 --pmc` (FETCH_SIZE, then WRITE_SIZE; one counter per pass) on a few steps of the same
+    workload, precision, tile configuration and geometry kernels, graphs off.  Each counter is
+    calibrated on the 8-byte scale kernel of the child's own bandwidth probe (`hc_stream_triad`:
+    `triad_n` doubles read and `triad_n` doubles written), i.e. bytes = counter x (8 triad_n / counter
+    of k_scale8).  Returns (bytes, details) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
+        return None, "rocprofv3 not found"
+    rf = "float" if precision else "double"
+    kib = {}
+    calib = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out = tempfile.mkdtemp(prefix="hc_traffic_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "-d", out, "-o", "run", "--output-format", "csv", "--",
+               sys.executable, os.path.abspath(__file__), "--workload", a.workload, "--steps", "2",
+               "--warmup", "0", "--graph-steps", "0", "--no-cpu-baseline", "--no-extras",
+               "--precision", "mixed" if precision else "fp64", "--tiles", str(tile_cfg),
+               "--uniform-geometry", str(a.uniform_geometry), "--tile-order", str(a.tile_order),
+               "--traffic", "file", "--counter-pass"]
+        try:
+            r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, timeout=timeout_s,
+                               env=dict(os.environ, TMPDIR="/tmp"), cwd="/tmp")
+            rows = []
+            for dirpath, _, files in os.walk(out):
+                for fn in files:
+                    if fn.endswith("counter_collection.csv"):
+                        with open(os.path.join(dirpath, fn)) as f:
+                            rows += list(csv.DictReader(f))
+        except (OSError, subprocess.SubprocessError, csv.Error) as e:
+            shutil.rmtree(out, ignore_errors=True)
+            return None, f"{counter} pass failed: {e}"
+        shutil.rmtree(out, ignore_errors=True)
+        got = stage_counter(rows, counter, rf, triad_n)
+        if got is None:
+            return None, f"{counter} pass: no stage or calibration dispatches (exit {r.returncode})"
+        name, kib[counter], calib[counter] = got
+    bytes_ = 1024.0 * (kib["FETCH_SIZE"] * calib["FETCH_SIZE"] + kib["WRITE_SIZE"] * calib["WRITE_SIZE"])
+    return bytes_, {"kernel": name, "FETCH_SIZE_KiB": kib["FETCH_SIZE"], "WRITE_SIZE_KiB": kib["WRITE_SIZE"],
+                    "fetch_scale": calib["FETCH_SIZE"], "write_scale": calib["WRITE_SIZE"]}
+
+
+def stage_counter(rows, counter, rf, triad_n):
+    """From rocprofv3 counter rows (dicts with Kernel_Name, Counter_Name, Counter_Value): the stage
+    kernel of element type `rf` with the most dispatches, its mean counter value in KiB, and the
+    calibration factor 8 triad_n / 1024 / (mean counter of k_scale8, which moves triad_n doubles
+    each way).  None when either kernel is missing."""
+    per = {}
+    for row in rows:
+        if row.get("Counter_Name") != counter:
+            continue
+        per.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    stage = [(len(v), k) for k, v in per.items() if k.startswith("void hc::k_stage<") and f", {rf}, " in k]
+    scale = [v for k, v in per.items() if "k_scale8" in k]
+    if not stage or not scale or float(np.mean(scale[0])) <= 0:
+        return None
+    name = max(stage)[1]
+    return name, float(np.mean(per[name])), triad_n * 8.0 / 1024.0 / float(np.mean(scale[0]))
+
+
+def traffic_entry(a, workload, precision, tile_cfg, uniform, triad_n):
+    """roofline.traffic: measured live (traffic_live) when a.traffic is "live", else (and as the
+    fall-back) the committed figure of profiles/traffic.json."""
+    committed = traffic_from_profiles(workload, precision, tile_cfg, uniform)
+    if a.traffic == "live":
+        live, info = traffic_live(a, precision, tile_cfg, triad_n)
+        if live is not None:
+            print(f"# traffic (rocprofv3 child runs): {live:.4e} B per launch, {info}", file=sys.stderr, flush=True)
+            return live, {"source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE child runs of this run",
+                          "committed_profile_bytes": committed, **info}
+        print(f"# traffic: live measurement unavailable ({info}); committed profile figure used",
+              file=sys.stderr, flush=True)
+        return committed, {"source": "profiles/traffic.json", "live_error": info}
+    return committed, {"source": "profiles/traffic.json"}
+
+
 def batch_cfgs(flux=None):
     """The 96-case cylinder batch (table drawn with MT19937-64 seed 20261202)."""
     from hyperboliconvergence_amd import cases
@@ -187,7 +276,7 @@ def batch_cfgs(flux=None):
             for r in cases.batch_case_table()]
 
 
-def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True, tile_order=1):
+def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True, tile_order=1, tiles=-1):
     """Time `steps` steps of the batch from the initial states after `warmup` untimed steps.
     Set-up outside the timed region and outside the warm-up count: the tile-shape autotuning and
     the capture of the step graph (one graph's worth of steps, then the states are reset)."""
@@ -199,6 +288,7 @@ def run_batch(cfgs, precision, steps, warmup, graph_steps, keep_states=True, til
     b.end()
     b.set_precision(precision)
     b.set_tile_order(tile_order)
+    b.set_tiles(tiles)
     ngeo = b.geometry_blocks()
     k = min(graph_steps, steps) if graph_steps > 0 else 0
     b.set_graph_steps(k)
@@ -322,7 +412,7 @@ def main_batch(a):
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
         el, qs, st, region, tiles, ngeo = run_batch(cfgs, prec, steps, a.warmup, a.graph_steps,
-                                                    tile_order=a.tile_order)
+                                                    tile_order=a.tile_order, tiles=a.tiles)
         res[prec] = dict(seconds=el, qs=qs, status=st, kernel_ms=region[0] / max(region[1], 1.0),
                          rate=Ntot * ns * steps / el, tiles=int(tiles[0]), tile_ms=[float(x) for x in tiles[1]])
         print(f"# batch {'mixed' if prec else 'fp64'}: {steps} steps in {el:.4f} s, {res[prec]['rate']:.4e} "
@@ -330,7 +420,7 @@ def main_batch(a):
               f"tile shape {res[prec]['tiles']} (ms/step {res[prec]['tile_ms']})", file=sys.stderr, flush=True)
     # mixed against FP64 at the configured horizon, whatever the timed step count
     per_case, mixed_err = None, None
-    if 1 in res:
+    if 1 in res and not a.counter_pass:
         if steps == HORIZON_C4 and 0 in res:
             q64, q32 = res[0]["qs"], res[1]["qs"]
         else:
@@ -354,8 +444,9 @@ def main_batch(a):
             ref["mixed_status" if prec else "fp64_status"] = int(st)
         print(f"# batch reference Roe over 50 steps: fp64 {ref['fp64_value']:.4e}, mixed {ref['mixed_value']:.4e} "
               f"cell-stage/s", file=sys.stderr, flush=True)
+    triad_n = TRIAD_DOUBLES["c4"]
     with hca.Context(0) as c:
-        triad = c.stream_triad(1 << 27, 20)
+        triad = c.stream_triad(triad_n, 20)
     # compulsory bytes per cell-stage of a curvilinear case: the stage state read, the step-start
     # state read and the new state written (3 x 4 x 8 B) plus one FP64 node of geometry (16 B).
     # Cases on the same mesh share one geometry copy, so counting the geometry once per distinct
@@ -365,8 +456,10 @@ def main_batch(a):
     geo_cells = sum(s0 * s1 for s0, s1 in shapes) if ngeo == len(shapes) else Ntot
     bpc_shared = 96.0 + 16.0 * geo_cells / Ntot
     achieved = bpc * Ntot / (h["kernel_ms"] * 1e-3) / 1e9
+    traffic, traffic_info = traffic_entry(a, "c4", head, h["tiles"], False, triad_n)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles("c4", head, h["tiles"]),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_measurement": traffic_info,
+                "traffic_bytes_per_cell_stage": traffic / Ntot if traffic else None,
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bpc, "kernel_us": h["kernel_ms"] * 1e3,
                 "alg_bytes_per_cell_stage_shared_mesh": bpc_shared,
                 "frac_shared_mesh": bpc_shared * Ntot / (h["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
@@ -526,6 +619,32 @@ def main_adjoint(a):
     print(json.dumps(out), flush=True)
 
 
+def launch_with_live_traffic(a):
+    """The measured run in a child process of this script (which keeps the committed traffic
+    figure), then the two rocprofv3 counter passes on the tile configuration and precision that
+    run reported, and the child's bench line with roofline.traffic replaced by the measured
+    bytes.  This process never touches the GPU: the profiler starts its program by exec, which
+    must not come from a process that has initialised the GPU."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__)] + sys.argv[1:] + ["--traffic", "file"],
+                       stdout=subprocess.PIPE, text=True)
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        sys.stdout.write(r.stdout)
+        sys.exit(r.returncode or 1)
+    out = json.loads(lines[-1])
+    cfg, roof = out["config"], out.get("roofline")
+    if roof is not None:
+        prec = 1 if cfg["precision"] == "mixed" else 0
+        uni = bool(cfg.get("uniform_geometry", False))
+        traffic, info = traffic_entry(a, a.workload, prec, cfg["tile_shape"], uni,
+                                      TRIAD_DOUBLES.get(a.workload, 1 << 26))
+        roof["traffic"] = traffic
+        roof["traffic_measurement"] = info
+        roof["traffic_bytes_per_cell_stage"] = traffic / cfg["cells"] if traffic else None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -541,6 +660,14 @@ def main():
                     help="c4: 1 interleaves the tiles of cases sharing a mesh in a step, 0 runs case by case")
     ap.add_argument("--uniform-geometry", type=int, default=1, choices=[0, 1],
                     help="1: uniform-geometry stage kernels where the mesh allows them (c1, c3); 0: general kernels")
+    ap.add_argument("--tiles", type=int, default=-1, choices=[-1, 0, 1, 2],
+                    help="stage-kernel tile configuration: -1 autotune on the first steps, else fixed")
+    ap.add_argument("--counter-pass", action="store_true",
+                    help="a child run under rocprofv3: the timed steps and the bandwidth probe only, "
+                         "no precision comparison")
+    ap.add_argument("--traffic", default="live", choices=["live", "file"],
+                    help="roofline.traffic: live = two rocprofv3 --pmc child runs of this script (FETCH_SIZE, "
+                         "WRITE_SIZE), file = the committed profiles/traffic.json figure")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -550,6 +677,8 @@ def main():
         print("bench.py runs on one GPU only", file=sys.stderr)
         sys.exit(2)
 
+    if a.traffic == "live" and a.workload != "c5" and not a.counter_pass:
+        return launch_with_live_traffic(a)
     if a.workload == "c4":
         return main_batch(a)
     if a.workload == "c5":
@@ -567,7 +696,7 @@ def main():
     res = {}
     for prec in ((0, 1) if a.precision == "both" else ((0,) if a.precision == "fp64" else (1,))):
         el, q, st, (region, per_launch, tiles, tsim, uni), warmup = run_gpu(cfg, prec, steps, a.warmup, a.graph_steps,
-                                                                            a.uniform_geometry)
+                                                                            a.uniform_geometry, a.tiles)
         # stage-kernel time: the events' interval over the timed steps per stage launch (it also
         # holds the global-dt update kernels and the gaps between launches: an upper bound)
         kern_ms = region[0] / max(region[1], 1.0)
@@ -632,13 +761,16 @@ def main():
               "mixed_vs_fp64_rel_err": ff_err, "status": [int(ff[0]["status"]), int(ff[1]["status"])]}
 
     import hyperboliconvergence_amd as hca
+    triad_n = TRIAD_DOUBLES.get(a.workload, 1 << 26)
     with hca.Context(0) as c:
-        triad = c.stream_triad(1 << 26, 20)
+        triad = c.stream_triad(triad_n, 20)
     alg_bytes = bytes_per_cell_stage * N
     achieved = alg_bytes / (h["kernel_ms"] * 1e-3) / 1e9
+    traffic, traffic_info = traffic_entry(a, a.workload, head, h["tiles"], h["uni"], triad_n)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS,
-                "traffic": traffic_from_profiles(a.workload, head, h["tiles"], h["uni"]),
+                "traffic": traffic, "traffic_measurement": traffic_info,
+                "traffic_bytes_per_cell_stage": traffic / N if traffic else None,
                 "attainable_triad_GBs": triad, "alg_bytes_per_cell_stage": bytes_per_cell_stage,
                 "kernel_us": h["kernel_ms"] * 1e3, "kernel_us_eager_events": h["launch_ms"] * 1e3,
                 "update_kernel_us": h["upd_ms"] * 1e3 if cs.dt_mode == 0 else None}
