@@ -677,7 +677,10 @@ def main():
         print("bench.py runs on one GPU only", file=sys.stderr)
         sys.exit(2)
 
-    if a.traffic == "live" and a.workload != "c5" and not a.counter_pass:
+    # under rocprofv3 (which exports ROCPROF_OUTPUT_PATH to its program) the profiler's library has
+    # initialised the GPU in this process, which then must not start the profiler again: the
+    # committed traffic figure is reported instead
+    if a.traffic == "live" and a.workload != "c5" and not a.counter_pass and "ROCPROF_OUTPUT_PATH" not in os.environ:
         return launch_with_live_traffic(a)
     if a.workload == "c4":
         return main_batch(a)

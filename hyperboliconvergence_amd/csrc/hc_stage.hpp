@@ -887,20 +887,45 @@ k_stage(LaunchArgs<RF> L)
     const TileRef tr = L.tiles[L.tile_base + xcd_tile(blockIdx.x, L.ntiles)];
     const int ci = tr.ci, lt = tr.lt;
     const CaseRec<RF>& c = L.cases[ci];
-    const StageArgs<RF> a = case_args<RF, T::NV>(L, c, ci);
     const int i0 = (lt / c.tiles_j) * TI, j0 = (lt % c.tiles_j) * TJ;
+    // The case's parameters (18 array pointers, species, Runge-Kutta plan) held from the first
+    // phase to the last exceed the scalar register file, and the general-geometry kernels then
+    // spill them into vector-register lanes (about 110 lane moves per wavefront).  There, each
+    // group of phases reads them again from the case record and the kernel arguments (scalar-cache
+    // hits) through pointers passed through an empty asm, which hides that they are the ones read
+    // before, so the values of one group die at its barrier.  The uniform-geometry kernels read
+    // them once (the reads per group cost those kernels registers and time).
+    auto args = [&]() -> StageArgs<RF> {
+        if constexpr (UNI != 0) {
+            return case_args<RF, T::NV>(L, c, ci);
+        } else {
+            using ConstRec = const __attribute__((address_space(4))) CaseRec<RF>;
+            using ConstArgs = const __attribute__((address_space(4))) LaunchArgs<RF>;
+            ConstRec* cr = (ConstRec*)&L.cases[__builtin_amdgcn_readfirstlane(ci)];
+            ConstArgs* la = (ConstArgs*)__builtin_amdgcn_kernarg_segment_ptr();   // L: the only argument, at offset 0
+            asm volatile("" : "+s"(cr), "+s"(la) : : "memory");
+            return case_args<RF, T::NV>(*(const LaunchArgs<RF>*)la, *(const CaseRec<RF>*)cr, ci);
+        }
+    };
     unsigned int flag = 0;
-    T::load(a, s, i0, j0, tid);
     typename T::Geo g0;
-    T::prefetch(a, i0, j0, tid, g0);
-    __syncthreads();
-    T::qmetric(a, s, i0, j0, tid);
-    __syncthreads();
     typename T::CellIn cin;
-    T::prefetch_cell(a, i0, j0, tid, cin);
-    T::faces(a, s, i0, j0, tid, flag, g0);
+    {
+        const StageArgs<RF> a = args();
+        T::load(a, s, i0, j0, tid);
+        T::prefetch(a, i0, j0, tid, g0);
+        __syncthreads();
+        T::qmetric(a, s, i0, j0, tid);
+    }
+    __syncthreads();
+    {
+        const StageArgs<RF> a = args();
+        T::prefetch_cell(a, i0, j0, tid, cin);
+        T::faces(a, s, i0, j0, tid, flag, g0);
+    }
     __syncthreads();
     double red;
+    const StageArgs<RF> a = args();
     T::cells(a, s, i0, j0, tid, flag, red, cin);
     // reductions: the status bits; for global-dt stage 0 also max(lambda/vol), over the
     // work-group and into the tile's slot (mode is uniform over the grid: the barrier is too)
