@@ -682,6 +682,7 @@ def main():
     # committed traffic figure is reported instead
     if a.traffic == "live" and a.workload != "c5" and not a.counter_pass and "ROCPROF_OUTPUT_PATH" not in os.environ:
         return launch_with_live_traffic(a)
+    a.traffic = "file"   # the process that measures never starts the profiler itself
     if a.workload == "c4":
         return main_batch(a)
     if a.workload == "c5":
