@@ -160,6 +160,17 @@ HC_HD bool hc_isfinite(double x) { return (dbl_bits(x) & 0x7FF0000000000000ULL) 
 
 // Bijection of work-group ids onto tiles such that the ids an XCD receives
 // (b % 8 == x) cover one contiguous band of tiles.
+// the value of the first active lane, which the compiler then keeps in a scalar register: only
+// for values that are equal on every active lane of the wavefront (identity on the host)
+HC_HD int wave_uniform(int v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_readfirstlane(v);
+#else
+    return v;
+#endif
+}
+
 HC_HD int xcd_tile(int b, int ntiles)
 {
     const int q = ntiles / 8, r = ntiles % 8;
@@ -488,15 +499,35 @@ struct StageTile {
     // Geometry prefetch: the slots of the lane's first face (k = tid), loaded while the tile and
     // the metric phase are in flight (their latency otherwise opens the face phase); the centre
     // difference of centre staging is filled in the face phase
+    // Face k of the tile: its kind and its position (li, lj) in the kind's layout.  When the i-faces
+    // fill whole wavefronts (NFI a multiple of 64), the 64 consecutive k of a wavefront are of one
+    // kind, so the kind is wave-uniform: the operands it selects (face-slot arrays, strides,
+    // periods, LDS offsets) are then picked with scalar selects instead of per-lane ones.
+    static constexpr bool KIND_UNIFORM = NFI % 64 == 0 && NT % 64 == 0;
+    HC_HD static bool face_kind_i(int k)
+    {
+        if constexpr (KIND_UNIFORM) return wave_uniform(k) < NFI;
+        else return k < NFI;
+    }
+    HC_HD static void face_pos(int k, bool isi, int& li, int& lj)
+    {
+        if (isi) {
+            li = k / TJ;
+            lj = k % TJ;
+        } else {
+            li = (k - NFI) / (TJ + 1);
+            lj = (k - NFI) % (TJ + 1);
+        }
+    }
+
     HC_HD static void prefetch(const StageArgs<RF>& a, int i0, int j0, int tid, Geo& g)
     {
         if constexpr (GPF != 0 && UNI == 0) {
             const int k = tid;
             if (k >= NF) return;
-            const bool isi = k < NFI;
-            const int kk = isi ? k : k - NFI;
-            const int w = isi ? TJ : TJ + 1;
-            const int li = kk / w, lj = kk % w;
+            const bool isi = face_kind_i(k);
+            int li, lj;
+            face_pos(k, isi, li, lj);
             const int l = isi ? i0 + li : j0 + lj;
             const int t = isi ? j0 + lj : i0 + li;
             const int lmax = isi ? a.m.ni : a.m.nj;
@@ -517,10 +548,9 @@ struct StageTile {
         const QT* qs = &s.q[0][0][0];
         const RF* Qs = &s.Q[0][0][0][0];
         for (int k = tid; k < NF; k += NT) {
-            const bool isi = k < NFI;
-            const int kk = isi ? k : k - NFI;
-            const int w = isi ? TJ : TJ + 1;
-            const int li = kk / w, lj = kk % w;
+            const bool isi = face_kind_i(k);
+            int li, lj;
+            face_pos(k, isi, li, lj);
             const int l = isi ? i0 + li : j0 + lj;        // node line of the face
             const int t = isi ? j0 + lj : i0 + li;        // transverse cell index
             const int lmax = isi ? ni : nj;
