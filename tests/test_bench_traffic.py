@@ -61,3 +61,48 @@ def test_traffic_entry_falls_back_to_the_committed_figure(monkeypatch):
     monkeypatch.setattr(b, "traffic_live", lambda a, p, t, n: (123.0, {"kernel": F64}))
     v, info = b.traffic_entry(A(), "c4", 0, 2, False, 1 << 27)
     assert v == 123.0 and info["committed_profile_bytes"] == b.traffic_from_profiles("c4", 0, 2)
+
+
+def test_launcher_replaces_the_childs_traffic_with_the_live_figure(monkeypatch, capsys):
+    import json
+    import subprocess
+    import sys
+    b = _bench()
+    line = {"metric": "m", "value": 1.0, "config": {"precision": "fp64", "tile_shape": 2, "cells": 1000},
+            "roofline": {"traffic": 5.0}}
+    calls = []
+
+    def fake_run(cmd, **kw):
+        calls.append(cmd)
+        return subprocess.CompletedProcess(cmd, 0, stdout="# progress\n" + json.dumps(line) + "\n")
+
+    monkeypatch.setattr(subprocess, "run", fake_run)
+    monkeypatch.setattr(b, "traffic_live", lambda a, p, t, n: (2000.0, {"kernel": F64}))
+    monkeypatch.setattr(sys, "argv", ["bench.py", "--workload", "c4"])
+
+    class A:
+        traffic = "live"
+        workload = "c4"
+
+    b.launch_with_live_traffic(A())
+    out = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    assert calls and calls[0][-2:] == ["--traffic", "file"]   # the child keeps the committed figure
+    assert out["roofline"]["traffic"] == 2000.0
+    assert out["roofline"]["traffic_bytes_per_cell_stage"] == 2.0
+    assert out["roofline"]["traffic_measurement"]["kernel"] == F64
+
+
+def test_launcher_passes_a_failed_measurement_through(monkeypatch, capsys):
+    import subprocess
+    import sys
+    b = _bench()
+    monkeypatch.setattr(subprocess, "run", lambda cmd, **kw: subprocess.CompletedProcess(cmd, 3, stdout=""))
+    monkeypatch.setattr(sys, "argv", ["bench.py"])
+
+    class A:
+        traffic = "live"
+        workload = "c4"
+
+    with pytest.raises(SystemExit) as e:
+        b.launch_with_live_traffic(A())
+    assert e.value.code == 3
