@@ -616,6 +616,13 @@ struct StageTile {
     // cylinder 3.5 %), while AUSM+up needs more than 96 either way (four waves), so only its
     // kernels keep the prefetch (the 4096^2 vortex runs 3.7 % slower without it).
     static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double) && LAW == LAW_EULER && FLUX == FLUX_AUSMPUP;
+    // The other FP64 prefetching configurations load the same inputs at the end of the face phase,
+    // once the lane's face is stored: the face phase's registers are free by then, and the loads are
+    // in flight while the work-group waits at the barrier before the cell phase.  (Not in the mixed
+    // objects, which are built with FMA contraction: a different code shape in some tile
+    // configurations could contract the update differently from the others.)
+    static constexpr bool LPF = GPF != 0 && !CPF && sizeof(RF) == sizeof(double);
+    static constexpr bool PF = CPF || LPF;
     struct CellIn {
         double q0[NV], acc[NV], vol, lam0;
     };
@@ -631,7 +638,7 @@ struct StageTile {
 
     HC_HD static void prefetch_cell(const StageArgs<RF>& a, int i0, int j0, int tid, CellIn& ci)
     {
-        if constexpr (CPF) {
+        if constexpr (PF) {
             const int i = i0 + tid / TJ, j = j0 + tid % TJ;
             if (tid >= NC || i >= a.m.ni || j >= a.m.nj || a.mode == MODE_RESIDUAL) return;
             const long long c = (long long)i * a.m.nj + j;
@@ -696,7 +703,7 @@ struct StageTile {
         double Rt[NV];
         for (int v = 0; v < NV; ++v) {
             double base = 0.0;
-            if (p.base >= 0) base = CPF ? ci.acc[v] : a.acc[((long long)p.base * NV + v) * a.N + c];
+            if (p.base >= 0) base = PF ? ci.acc[v] : a.acc[((long long)p.base * NV + v) * a.N + c];
             Rt[v] = base + R[v] * p.a_ss;
         }
         // unrolled over the plan's fixed capacity: constant indices only (the arguments may live in registers)
@@ -706,11 +713,11 @@ struct StageTile {
             for (int v = 0; v < NV; ++v) {
                 double* A = &a.acc[((long long)p.upd_slot[u] * NV + v) * a.N + c];
                 double old = 0.0;
-                if (!p.upd_first[u]) old = (CPF && u == 0 && p.base < 0) ? ci.acc[v] : *A;
+                if (!p.upd_first[u]) old = (PF && u == 0 && p.base < 0) ? ci.acc[v] : *A;
                 *A = old + R[v] * p.upd_coef[u];
             }
         }
-        const double vol = CPF ? ci.vol : a.m.cvol[UNI ? 0 : c];
+        const double vol = PF ? ci.vol : a.m.cvol[UNI ? 0 : c];
         if (a.mode == MODE_GLOBAL_STAGE0) {
             for (int v = 0; v < NV; ++v) a.tmp[v * a.N + c] = Rt[v];
             red = lam / vol;   // spectralRadius.h:33-35
@@ -728,12 +735,12 @@ struct StageTile {
                 l0 = lam;
                 a.lam0[c] = lam;
             } else {
-                l0 = CPF ? ci.lam0 : a.lam0[c];
+                l0 = PF ? ci.lam0 : a.lam0[c];
             }
             fac = (p.beta * a.cfl) / l0;
         }
         double q0[NV];
-        for (int v = 0; v < NV; ++v) q0[v] = CPF ? ci.q0[v] : a.q0[v * a.N + c];
+        for (int v = 0; v < NV; ++v) q0[v] = PF ? ci.q0[v] : a.q0[v * a.N + c];
         update_cell(a, c, q0, Rt, fac, flag);
     }
 
@@ -950,8 +957,9 @@ k_stage(LaunchArgs<RF> L)
     __syncthreads();
     {
         const StageArgs<RF> a = args();
-        T::prefetch_cell(a, i0, j0, tid, cin);
+        if constexpr (T::CPF) T::prefetch_cell(a, i0, j0, tid, cin);
         T::faces(a, s, i0, j0, tid, flag, g0);
+        if constexpr (T::LPF) T::prefetch_cell(a, i0, j0, tid, cin);
     }
     __syncthreads();
     double red;
