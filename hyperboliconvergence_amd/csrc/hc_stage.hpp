@@ -611,11 +611,12 @@ struct StageTile {
     // first, volume, stage-0 spectral radius) are loaded into registers when the face phase opens,
     // so the cell phase does not start with a round trip to memory.  The 20 registers it holds
     // through the face phase decide the occupancy of the FP64 7x16 kernels: without it the
-    // central, Rusanov, Roe-family and SLAU kernels fit 96 VGPRs (five waves per SIMD, which the
-    // 31 KB of LDS per work-group also allows; the 96-case batch runs 4.4 % faster, the 512x256
-    // cylinder 3.5 %), while AUSM+up needs more than 96 either way (four waves), so only its
-    // kernels keep the prefetch (the 4096^2 vortex runs 3.7 % slower without it).
-    static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double) && LAW == LAW_EULER && FLUX == FLUX_AUSMPUP;
+    // central, Rusanov, Roe-family and SLAU kernels, and AUSM+up in the conserved basis, fit 96
+    // VGPRs (five waves per SIMD, which the 31 KB of LDS per work-group also allows), while
+    // AUSM+up in the primitive basis needs more than 96 either way (four waves), so only that
+    // kernel keeps the early prefetch (the 4096^2 vortex runs 3 % slower with the late one).
+    static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double) && LAW == LAW_EULER &&
+                                FLUX == FLUX_AUSMPUP && BASIS == BASIS_PRIMITIVE;
     // The other FP64 prefetching configurations load the same inputs at the end of the face phase,
     // once the lane's face is stored: the face phase's registers are free by then, and the loads are
     // in flight while the work-group waits at the barrier before the cell phase.  (Not in the mixed
