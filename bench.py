@@ -725,7 +725,7 @@ def main():
         mixed_err = rel_err_conserved(q32, q64, cs.law, cs.basis)
         print(f"# mixed vs fp64 state error after {err_steps} steps: {mixed_err:.3e} "
               f"(fp64 first non-finite at step {diverge})", file=sys.stderr, flush=True)
-    head = 1 if (1 in res and mixed_err is not None and mixed_err <= 1e-6) else 0
+    head = 1 if (1 in res and mixed_err is not None and mixed_err <= 1e-6) else (0 if 0 in res else 1)
     h = res[head]
 
     # isentropic vortex: L2 density error against the analytic solution at the physical time

@@ -617,12 +617,15 @@ struct StageTile {
     // kernel keeps the early prefetch (the 4096^2 vortex runs 3 % slower with the late one).
     static constexpr bool CPF = GPF != 0 && sizeof(RF) == sizeof(double) && LAW == LAW_EULER &&
                                 FLUX == FLUX_AUSMPUP && BASIS == BASIS_PRIMITIVE;
-    // The other FP64 prefetching configurations load the same inputs at the end of the face phase,
-    // once the lane's face is stored: the face phase's registers are free by then, and the loads are
-    // in flight while the work-group waits at the barrier before the cell phase.  (Not in the mixed
-    // objects, which are built with FMA contraction: a different code shape in some tile
-    // configurations could contract the update differently from the others.)
-    static constexpr bool LPF = GPF != 0 && !CPF && sizeof(RF) == sizeof(double);
+    // The other FP64 prefetching configurations, and every configuration of the mixed objects but
+    // AUSM+up's, load the same inputs at the end of the face phase, once the lane's face is stored:
+    // the face phase's registers are free by then, and the loads are in flight while the work-group
+    // waits at the barrier before the cell phase (the mixed AUSM+up kernels, FP32 faces throughout,
+    // run 3 % slower with it).  The mixed objects are built with FMA contraction, so the choice is
+    // made per flux for all tile configurations alike: a cell phase of another shape in some of them
+    // could contract the update differently, and cases would then depend on the tile shape in the
+    // last bits.
+    static constexpr bool LPF = (GPF != 0 && !CPF) || (sizeof(RF) == sizeof(float) && FLUX != FLUX_AUSMPUP);
     static constexpr bool PF = CPF || LPF;
     struct CellIn {
         double q0[NV], acc[NV], vol, lam0;
