@@ -930,14 +930,15 @@ k_stage(LaunchArgs<RF> L)
     const CaseRec<RF>& c = L.cases[ci];
     const int i0 = (lt / c.tiles_j) * TI, j0 = (lt % c.tiles_j) * TJ;
     // The case's parameters (18 array pointers, species, Runge-Kutta plan) held from the first
-    // phase to the last exceed the scalar register file, and the general-geometry kernels then
-    // spill them into vector-register lanes (about 110 lane moves per wavefront).  There, each
-    // group of phases reads them again from the case record and the kernel arguments (scalar-cache
-    // hits) through pointers passed through an empty asm, which hides that they are the ones read
-    // before, so the values of one group die at its barrier.  The uniform-geometry kernels read
-    // them once (the reads per group cost those kernels registers and time).
-    auto args = [&]() -> StageArgs<RF> {
-        if constexpr (UNI != 0) {
+    // phase to the last exceed the scalar register file, and the kernels then spill them into
+    // vector-register lanes (about 110 lane moves per wavefront).  Instead, a group of phases reads
+    // them again from the case record and the kernel arguments (scalar-cache hits) through pointers
+    // passed through an empty asm, which hides that they are the ones read before, so the values
+    // of the previous group die at its barrier: the general-geometry kernels read them for the
+    // face phase and again for the cell phase, the uniform-geometry kernels once more for the cell
+    // phase only (a read for their face phase costs them vector registers and time).
+    auto args = [&](bool reread = UNI == 0) -> StageArgs<RF> {
+        if (!reread) {
             return case_args<RF, T::NV>(L, c, ci);
         } else {
             using ConstRec = const __attribute__((address_space(4))) CaseRec<RF>;
@@ -967,7 +968,7 @@ k_stage(LaunchArgs<RF> L)
     }
     __syncthreads();
     double red;
-    const StageArgs<RF> a = args();
+    const StageArgs<RF> a = args(true);
     T::cells(a, s, i0, j0, tid, flag, red, cin);
     // reductions: the status bits; for global-dt stage 0 also max(lambda/vol), over the
     // work-group and into the tile's slot (mode is uniform over the grid: the barrier is too)
